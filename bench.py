@@ -1,0 +1,219 @@
+"""Benchmark of the MI355X dedispersion engine (one JSON line on stdout).
+
+A step is one pass of the hot path over one batch with the input already
+resident in HBM.  The workload and metric are DM-trial·samples per second:
+one unit is one element of the final dedispersed output (beam x DM trial x
+sample).  The dominant kernel's average duration is measured with HIP events
+recorded on the stream the kernels run on, and turned into an HBM-roofline
+fraction from its algorithmic bytes.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # bytes/s, MI355X HBM3E
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="arts-two-step",
+                    choices=["arts-two-step", "arts-single"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
+    return ap.parse_args()
+
+
+class EventTimer:
+    """Per-kernel durations from HIP events on one stream."""
+
+    def __init__(self, torch):
+        self.torch = torch
+        self.pairs = {}
+
+    def around(self, name, fn):
+        t = self.torch
+        a, b = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+        a.record()
+        fn()
+        b.record()
+        self.pairs.setdefault(name, []).append((a, b))
+
+    def averages_ms(self):
+        return {k: sum(a.elapsed_time(b) for a, b in v) / len(v) for k, v in self.pairs.items()}
+
+
+def setup_two_step(torch, w):
+    from dedispersion_amd import Plan, synth_gaussian_u8
+
+    plan = Plan(w.obs, 128)
+    flags = w.zapped_flags() + [0] * (plan.pad(w.obs.nr_channels, 4) - w.obs.nr_channels)
+    plan.set_zapped_channels(flags)
+    z = plan.sizes
+    s = torch.cuda.current_stream().cuda_stream
+    d_in = torch.zeros(z.sub_input_elements, dtype=torch.uint8, device="cuda")
+    synth_gaussian_u8(d_in.data_ptr(), w.obs.nr_beams * w.obs.nr_channels, z.nr_dispersed_samples_subbanding,
+                      z.sub_in_pitch, 128.0, 16.0, w.seed, s)
+    for p in w.pulses:
+        plan.inject_pulse(d_in.data_ptr(), z.sub_in_pitch, z.nr_dispersed_samples_subbanding, p.dm_value, p.t0,
+                          p.width, p.amplitude, s)
+    d_mid = torch.empty(z.intermediate_elements, dtype=torch.float32, device="cuda")
+    d_out = torch.empty(z.sub_output_elements, dtype=torch.float32, device="cuda")
+    o = w.obs
+    units = o.nr_synthesized_beams * o.nr_dms_subbanding * o.nr_dms * z.nr_samples
+    C_live = o.nr_channels - len(w.zapped)
+    bytes_one = o.nr_beams * o.nr_channels * z.sub_in_pitch + 4 * o.nr_beams * o.nr_dms_subbanding * o.nr_subbands * z.nr_samples_subbanding
+    bytes_two = 4 * o.nr_beams * o.nr_dms_subbanding * o.nr_subbands * z.nr_samples_subbanding + 4 * units
+    kernels = {
+        "step_one": (lambda: plan.step_one(d_in.data_ptr(), d_mid.data_ptr(), stream=s), bytes_one),
+        "step_two": (lambda: plan.step_two(d_mid.data_ptr(), d_out.data_ptr(), stream=s), bytes_two),
+    }
+    return plan, kernels, units, (d_in, d_mid, d_out)
+
+
+def setup_single(torch, w):
+    from dedispersion_amd import Plan, synth_gaussian_u8
+
+    plan = Plan(w.obs, 128)
+    flags = w.zapped_flags() + [0] * (plan.pad(w.obs.nr_channels, 4) - w.obs.nr_channels)
+    plan.set_zapped_channels(flags)
+    z = plan.sizes
+    s = torch.cuda.current_stream().cuda_stream
+    d_in = torch.zeros(z.input_elements, dtype=torch.uint8, device="cuda")
+    synth_gaussian_u8(d_in.data_ptr(), w.obs.nr_channels, z.nr_dispersed_samples, z.in_pitch, 128.0, 16.0, w.seed, s)
+    d_out = torch.empty(z.output_elements, dtype=torch.float32, device="cuda")
+    o = w.obs
+    units = o.nr_synthesized_beams * o.nr_dms * z.nr_samples
+    nbytes = o.nr_beams * o.nr_channels * z.in_pitch + 4 * units
+    kernels = {"single_step": (lambda: plan.dedisperse(d_in.data_ptr(), d_out.data_ptr(), stream=s), nbytes)}
+    return plan, kernels, units, (d_in, d_out)
+
+
+def cpu_baseline(w, mode: str, seconds: float):
+    """The oracle on the host cores over a bounded sample of the workload."""
+    import numpy as np
+
+    import oracle
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    orc = oracle.Oracle(w.obs, 128, threads=threads)
+    o = w.obs
+    rng = np.random.default_rng(0)
+    if mode == "two-step":
+        inp = rng.integers(0, 256, size=o.nr_beams * o.nr_channels * orc.sub_in_pitch, dtype=np.uint8)
+        flags = np.zeros(oracle.pad(o.nr_channels, 32), dtype=np.uint32)
+        flags[w.zapped] = 1
+        # calibrate on one coarse DM, then size the sample to ~seconds
+        t = time.perf_counter()
+        mid = orc.step_one(inp, flags, rows=(0, 1))
+        orc.step_two(mid.ravel(), rows=(0, o.nr_dms))
+        per = time.perf_counter() - t
+        n = max(1, min(o.nr_dms_subbanding, int(seconds / max(per, 1e-3))))
+        t = time.perf_counter()
+        mid = orc.step_one(inp, flags, rows=(0, n))
+        orc.step_two(mid.ravel(), rows=(0, n * o.nr_dms))
+        dt = time.perf_counter() - t
+        units = n * o.nr_dms * (o.nr_samples_per_batch // o.downsampling)
+        sample = f"{n} of {o.nr_dms_subbanding} coarse DMs (step one, all subbands, then step two, {o.nr_dms} fine DMs each) of {w.name}"
+    else:
+        inp = rng.integers(0, 256, size=o.nr_beams * o.nr_channels * orc.in_pitch, dtype=np.uint8)
+        flags = np.zeros(oracle.pad(o.nr_channels, 32), dtype=np.uint32)
+        flags[w.zapped] = 1
+        t = time.perf_counter()
+        orc.dedispersion(inp, flags, rows=(0, threads))
+        per = (time.perf_counter() - t) / threads
+        n = max(threads, min(o.nr_dms, int(seconds / max(per, 1e-4) * threads)))
+        t = time.perf_counter()
+        orc.dedispersion(inp, flags, rows=(0, n))
+        dt = time.perf_counter() - t
+        units = n * (o.nr_samples_per_batch // o.downsampling)
+        sample = f"{n} of {o.nr_dms} DM rows of {w.name}"
+    return {"value": units / dt, "unit": "DM-trial*samples/s", "cores": threads, "kind": "port", "sample": sample,
+            "seconds": round(dt, 3)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU per process; multi-GPU runs are not part of this build",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+
+    from dedispersion_amd import workloads
+
+    if args.workload == "arts-two-step":
+        w = workloads.arts_two_step()
+        plan, kernels, units, keep = setup_two_step(torch, w)
+        mode = "two-step"
+    else:
+        w = workloads.arts_single()
+        plan, kernels, units, keep = setup_single(torch, w)
+        mode = "single-step"
+    torch.cuda.synchronize()
+
+    timer = EventTimer(torch)
+    for _ in range(args.warmup):
+        for fn, _ in kernels.values():
+            fn()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record()
+    for _ in range(args.steps):
+        for name, (fn, _) in kernels.items():
+            timer.around(name, fn)
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = start.elapsed_time(stop) / args.steps
+    value = units / (ms / 1e3)
+    avg = timer.averages_ms()
+    dominant = max(avg, key=avg.get)
+    bytes_dom = kernels[dominant][1]
+    achieved = bytes_dom / (avg[dominant] / 1e3)
+    total_bytes = sum(b for _, b in kernels.values())
+    line = {
+        "metric": "DM-trial*samples/s",
+        "value": value,
+        "unit": "DM-trial*samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8-in/f32-acc",
+        "data": "synthetic (seeded Gaussian noise N(128,16) + 8 dispersed pulses, 5% zapped channels)",
+        "config": {"workload": w.name, "mode": mode, "beams": w.obs.nr_synthesized_beams,
+                   "channels": w.obs.nr_channels, "samples": w.obs.nr_samples_per_batch,
+                   "dms": units // w.obs.nr_samples_per_batch // w.obs.nr_synthesized_beams, "padding": 128},
+        "hbm_roofline_frac_path": total_bytes / (ms / 1e3) / HBM_PEAK,
+        "kernels_ms": avg,
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None},
+        "wall_s": wall,
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        line["cpu_baseline"] = cpu_baseline(w, mode, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

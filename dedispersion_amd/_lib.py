@@ -1,0 +1,144 @@
+"""ctypes binding of the C ABI declared in include/dedispersion_amd.h.
+
+The shared library libdedisp_amd.so is built in-tree (``make -C
+dedispersion_amd`` or ``__graft_entry__.build()``).  Loading fails loudly when
+it is missing: there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_float, c_int, c_uint32, c_uint64, c_void_p
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdedisp_amd.so")
+
+DD_SINGLE_STEP, DD_STEP_ONE, DD_STEP_TWO = 0, 1, 2
+
+STATUS_NAMES = {
+    0: "DD_OK",
+    1: "DD_ERR_INVALID_ARGUMENT",
+    2: "DD_ERR_UNSUPPORTED",
+    3: "DD_ERR_HIP",
+    4: "DD_ERR_OUT_OF_MEMORY",
+    5: "DD_ERR_CONFIGURATION",
+    6: "DD_ERR_NOT_READY",
+    7: "DD_ERR_IO",
+}
+
+
+class DDObservation(ctypes.Structure):
+    _fields_ = [
+        ("nr_beams", c_uint32),
+        ("nr_synthesized_beams", c_uint32),
+        ("nr_channels", c_uint32),
+        ("nr_subbands", c_uint32),
+        ("min_freq", c_float),
+        ("channel_bandwidth", c_float),
+        ("sampling_time", c_float),
+        ("downsampling", c_uint32),
+        ("nr_samples_per_batch", c_uint32),
+        ("nr_dms", c_uint32),
+        ("first_dm", c_float),
+        ("dm_step", c_float),
+        ("nr_dms_subbanding", c_uint32),
+        ("first_dm_subbanding", c_float),
+        ("dm_step_subbanding", c_float),
+        ("nr_samples_per_dispersed_batch", c_uint32),
+        ("nr_samples_per_batch_subbanding", c_uint32),
+        ("nr_samples_per_dispersed_batch_subbanding", c_uint32),
+    ]
+
+
+class DDConf(ctypes.Structure):
+    _fields_ = [
+        ("threads_d0", c_uint32),
+        ("threads_d1", c_uint32),
+        ("items_d0", c_uint32),
+        ("items_d1", c_uint32),
+        ("unroll", c_uint32),
+        ("local", c_uint32),
+        ("split_batches", c_uint32),
+    ]
+
+    def as_tuple(self):
+        return tuple(getattr(self, f) for f, _ in self._fields_)
+
+
+class DDSizes(ctypes.Structure):
+    _fields_ = [
+        ("nr_samples", c_uint32),
+        ("nr_dispersed_samples", c_uint32),
+        ("nr_samples_subbanding", c_uint32),
+        ("nr_dispersed_samples_subbanding", c_uint32),
+        ("in_pitch", c_uint64),
+        ("out_pitch", c_uint64),
+        ("sub_in_pitch", c_uint64),
+        ("sub_out_pitch", c_uint64),
+        ("input_elements", c_uint64),
+        ("output_elements", c_uint64),
+        ("sub_input_elements", c_uint64),
+        ("intermediate_elements", c_uint64),
+        ("sub_output_elements", c_uint64),
+        ("max_shift", c_uint32),
+        ("max_shift_step_one", c_uint32),
+        ("max_shift_step_two", c_uint32),
+    ]
+
+
+# name -> (restype, argtypes); every exported symbol of include/dedispersion_amd.h
+PROTOTYPES = {
+    "dd_last_error": (c_char_p, []),
+    "dd_status_string": (c_char_p, [c_int]),
+    "dd_abi_version": (c_uint32, []),
+    "dd_plan_create": (c_int, [POINTER(c_void_p), POINTER(DDObservation), c_uint32]),
+    "dd_plan_destroy": (c_int, [c_void_p]),
+    "dd_plan_get_sizes": (c_int, [c_void_p, POINTER(DDSizes)]),
+    "dd_plan_set_conf": (c_int, [c_void_p, c_int, POINTER(DDConf)]),
+    "dd_plan_get_conf": (c_int, [c_void_p, c_int, POINTER(DDConf)]),
+    "dd_plan_set_zapped_channels": (c_int, [c_void_p, POINTER(c_uint32)]),
+    "dd_plan_set_beam_mapping": (c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint32)]),
+    "dd_plan_set_shifts": (c_int, [c_void_p, POINTER(c_float), POINTER(c_float)]),
+    "dd_plan_get_shifts": (c_int, [c_void_p, POINTER(c_float), POINTER(c_float)]),
+    "dd_plan_get_integer_shifts": (c_int, [c_void_p, c_int, POINTER(c_uint32)]),
+    "dd_dedisperse": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_void_p]),
+    "dd_subband_step_one": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "dd_subband_step_two": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_void_p]),
+    "dd_synth_gaussian_u8": (c_int, [c_void_p, c_uint64, c_uint64, c_uint64, c_float, c_float, c_uint64, c_void_p]),
+    "dd_synth_uniform_u8": (c_int, [c_void_p, c_uint64, c_uint64, c_uint64, c_uint64, c_void_p]),
+    "dd_synth_inject_pulse": (
+        c_int,
+        [c_void_p, c_void_p, c_uint64, c_uint32, c_float, c_uint32, c_uint32, c_uint32, c_void_p],
+    ),
+}
+
+_lib = None
+
+
+class DDError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+def load() -> ctypes.CDLL:
+    """Loads libdedisp_amd.so once and declares every prototype."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is not built; run `make -C dedispersion_amd` or __graft_entry__.build()"
+        )
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status: int) -> None:
+    if status != 0:
+        raise DDError(status, load().dd_last_error().decode())

@@ -1,0 +1,638 @@
+// plan.cpp -- the plan object behind the C ABI (dedispersion_amd.h).
+//
+// A plan fixes the observation and the padding, owns the device-resident
+// tables and turns each reference launch (src/DedispersionTest.cpp:367-399)
+// into one HIP kernel launch with validated geometry.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace ddamd {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &message) { g_last_error = message; }
+
+static inline uint32_t padto(uint32_t v, uint32_t k) { return k == 0 ? v : ((v + k - 1) / k) * k; }
+static inline uint32_t roundup(uint32_t v, uint32_t k) { return ((v + k - 1) / k) * k; }
+
+template <typename T> struct DeviceBuffer {
+  T *ptr = nullptr;
+  size_t count = 0;
+  ~DeviceBuffer() { release(); }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    count = 0;
+  }
+  hipError_t allocate(size_t n) {
+    release();
+    count = n;
+    if (n == 0) return hipSuccess;
+    return hipMalloc(reinterpret_cast<void **>(&ptr), n * sizeof(T));
+  }
+  hipError_t upload(const std::vector<T> &host) {
+    hipError_t e = allocate(host.size());
+    if (e != hipSuccess || host.empty()) return e;
+    return hipMemcpy(ptr, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice);
+  }
+  hipError_t download(std::vector<T> &host) const {
+    host.resize(count);
+    if (count == 0) return hipSuccess;
+    return hipMemcpy(host.data(), ptr, count * sizeof(T), hipMemcpyDeviceToHost);
+  }
+};
+
+// Integer shift table of one mode, device copy and host copy.
+struct ShiftTable {
+  uint32_t rows = 0, dms = 0, pitch = 0;
+  DeviceBuffer<uint32_t> dev;
+  std::vector<uint32_t> host;
+  uint32_t at(uint32_t r, uint32_t d) const { return host[(size_t)r * pitch + d]; }
+  uint32_t max_shift() const {
+    uint32_t m = 0;
+    for (uint32_t r = 0; r < rows; ++r)
+      for (uint32_t d = 0; d < dms; ++d) m = std::max(m, at(r, d));
+    return m;
+  }
+  // Largest shift spread inside any DM tile of `tile` consecutive DMs.
+  uint32_t max_spread(uint32_t tile) const {
+    uint32_t m = 0;
+    for (uint32_t r = 0; r < rows; ++r)
+      for (uint32_t d0 = 0; d0 < dms; d0 += tile) {
+        uint32_t a = at(r, d0), b = at(r, std::min(d0 + tile - 1, pitch - 1));
+        m = std::max(m, b - a);
+      }
+    return m;
+  }
+};
+
+} // namespace ddamd
+
+using namespace ddamd;
+
+struct dd_plan {
+  dd_observation obs{};
+  uint32_t padding = 0;
+  int device = 0;
+  size_t max_lds = 65536;
+  bool two_step = false;
+  dd_sizes sizes{};
+  dd_conf conf[3]{};
+
+  DeviceBuffer<float> shifts, shifts2, relative;
+  std::vector<float> h_shifts, h_shifts2;
+  ShiftTable ish_single, ish_one, ish_two;
+
+  std::vector<uint32_t> h_zapped;  // flags [pad(C)]
+  DeviceBuffer<uint32_t> live_single, live_single_count, live_sub, live_sub_count;
+  DeviceBuffer<uint32_t> map_single, map_two;
+};
+
+namespace {
+
+dd_status fail(dd_status s, const std::string &msg) {
+  set_error(msg);
+  return s;
+}
+
+dd_status hip_fail(hipError_t e, const char *what) {
+  return fail(DD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define DD_HIP(call)                                                                                 \
+  do {                                                                                               \
+    hipError_t e_ = (call);                                                                          \
+    if (e_ != hipSuccess) return hip_fail(e_, #call);                                                \
+  } while (0)
+
+uint32_t cps(const dd_plan *p) { return p->obs.nr_channels / p->obs.nr_subbands; }
+
+// Observation getters with the semantics stated in include/Observation.hpp.
+float max_freq(const dd_observation &o) {
+  float span = (float)(o.nr_channels - 1) * o.channel_bandwidth;
+  return o.min_freq + span;
+}
+float subband_bandwidth(const dd_observation &o) {
+  return (float)(o.nr_channels / o.nr_subbands) * o.channel_bandwidth;
+}
+float subband_min_freq(const dd_observation &o) {
+  float span = (float)(o.nr_channels / o.nr_subbands - 1) * o.channel_bandwidth;
+  return o.min_freq + span;
+}
+float subband_max_freq(const dd_observation &o) {
+  float span = (float)(o.nr_subbands - 1) * subband_bandwidth(o);
+  return subband_min_freq(o) + span;
+}
+
+// src/DedispersionTest.cpp:129 / :145 / :146
+uint32_t dispersed(uint32_t samples, float shift0, float first, float step, uint32_t dms) {
+  float last = first + ((float)(uint32_t)(dms - 1) * step);
+  float total = (float)samples + (shift0 * last);
+  return (uint32_t)std::ceil(total);
+}
+
+dd_status build_integer(dd_plan *p, ShiftTable &t, const float *d_table, uint32_t rows, uint32_t dms,
+                        float first, float step) {
+  t.rows = rows;
+  t.dms = dms;
+  t.pitch = roundup(std::max(dms, 1u), 64) + 64;
+  DD_HIP(t.dev.allocate((size_t)rows * t.pitch));
+  if (dms == 0 || rows == 0) {
+    t.host.assign((size_t)rows * t.pitch, 0);
+    return DD_OK;
+  }
+  DD_HIP(launch_integer_shifts(d_table, rows, dms, t.pitch, first, step, t.dev.ptr, nullptr));
+  DD_HIP(hipDeviceSynchronize());
+  DD_HIP(t.dev.download(t.host));
+  (void)p;
+  return DD_OK;
+}
+
+dd_status rebuild_integer_tables(dd_plan *p) {
+  const dd_observation &o = p->obs;
+  dd_status s = build_integer(p, p->ish_single, p->shifts.ptr, o.nr_channels, o.nr_dms, o.first_dm, o.dm_step);
+  if (s != DD_OK) return s;
+  if (p->two_step) {
+    DD_HIP(p->relative.allocate(o.nr_channels));
+    DD_HIP(launch_relative_shifts(p->shifts.ptr, o.nr_channels, cps(p), p->relative.ptr, nullptr));
+    s = build_integer(p, p->ish_one, p->relative.ptr, o.nr_channels, o.nr_dms_subbanding, o.first_dm_subbanding,
+                      o.dm_step_subbanding);
+    if (s != DD_OK) return s;
+    s = build_integer(p, p->ish_two, p->shifts2.ptr, o.nr_subbands, o.nr_dms, o.first_dm, o.dm_step);
+    if (s != DD_OK) return s;
+  }
+  p->sizes.max_shift = p->ish_single.max_shift();
+  p->sizes.max_shift_step_one = p->two_step ? p->ish_one.max_shift() : 0;
+  p->sizes.max_shift_step_two = p->two_step ? p->ish_two.max_shift() : 0;
+  return DD_OK;
+}
+
+dd_status rebuild_live(dd_plan *p) {
+  const uint32_t C = p->obs.nr_channels;
+  std::vector<uint32_t> single, count1(1, 0);
+  for (uint32_t ch = 0; ch < C; ++ch)
+    if (p->h_zapped[ch] == 0) single.push_back(ch);
+  count1[0] = (uint32_t)single.size();
+  if (single.empty()) single.push_back(0);
+  DD_HIP(p->live_single.upload(single));
+  DD_HIP(p->live_single_count.upload(count1));
+  if (p->two_step) {
+    const uint32_t n = cps(p), nsb = p->obs.nr_subbands;
+    std::vector<uint32_t> sub((size_t)nsb * n, 0), counts(nsb, 0);
+    for (uint32_t sb = 0; sb < nsb; ++sb)
+      for (uint32_t ch = sb * n; ch < (sb + 1) * n; ++ch)
+        if (p->h_zapped[ch] == 0) sub[(size_t)sb * n + counts[sb]++] = ch;
+    DD_HIP(p->live_sub.upload(sub));
+    DD_HIP(p->live_sub_count.upload(counts));
+  }
+  return DD_OK;
+}
+
+dd_conf default_conf(dd_mode mode) {
+  dd_conf c{};
+  c.threads_d0 = 64;
+  c.threads_d1 = 4;
+  c.items_d0 = mode == DD_STEP_TWO ? 8 : 16;
+  c.items_d1 = 4;
+  c.unroll = 4;
+  c.local = 1;
+  c.split_batches = 0;
+  return c;
+}
+
+// Geometry of a mode under its configuration; validates limits.
+dd_status geometry_of(const dd_plan *p, dd_mode mode, const dd_conf &c, Geometry &g) {
+  if (c.split_batches) return fail(DD_ERR_UNSUPPORTED, "split batches mode is not implemented");
+  if (c.threads_d0 != 64) return fail(DD_ERR_CONFIGURATION, "threads_d0 must be 64 (one wavefront along samples)");
+  if (c.threads_d1 < 1 || c.threads_d1 > 8) return fail(DD_ERR_CONFIGURATION, "threads_d1 must be in [1, 8]");
+  if (c.unroll < 1 || c.unroll > 16) return fail(DD_ERR_CONFIGURATION, "unroll must be in [1, 16]");
+  const uint32_t tile = c.threads_d1 * c.items_d1;
+  if (tile > 64) return fail(DD_ERR_CONFIGURATION, "threads_d1 * items_d1 must not exceed 64 DMs");
+  g.threads_d1 = c.threads_d1;
+  g.items_d0 = c.items_d0;
+  g.items_d1 = c.items_d1;
+  g.local = c.local ? 1 : 0;
+  g.win = 0;
+  g.lds_bytes = 0;
+  const uint32_t ts = 64 * c.items_d0;
+  if (mode == DD_STEP_TWO) {
+    if (!rowsum_f32_supported(c.items_d0, c.items_d1))
+      return fail(DD_ERR_CONFIGURATION, "items_d0 must be 4, 8 or 16 and items_d1 1, 2, 4 or 8");
+    if (g.local) {
+      // lane reads floats [spread + lane + 64k] for k < items_d0
+      g.win = roundup(ts + p->ish_two.max_spread(tile), 4);
+      g.lds_bytes = c.unroll * g.win * 4;
+    }
+  } else {
+    if (!rowsum_u8_supported(c.items_d0, c.items_d1))
+      return fail(DD_ERR_CONFIGURATION, "items_d0 must be 4, 8 or 16 and items_d1 1, 2, 4 or 8");
+    if (g.local) {
+      // window start misalignment (< 4) + spread + the dword after each group
+      const ShiftTable &t = mode == DD_SINGLE_STEP ? p->ish_single : p->ish_one;
+      g.win = roundup(ts + t.max_spread(tile) + 8, 16);
+      g.lds_bytes = c.unroll * g.win;
+    }
+  }
+  if (g.lds_bytes > p->max_lds)
+    return fail(DD_ERR_CONFIGURATION, "LDS demand " + std::to_string(g.lds_bytes) + " bytes exceeds the device limit");
+  return DD_OK;
+}
+
+bool aligned16(const void *ptr) { return (reinterpret_cast<uintptr_t>(ptr) & 15) == 0; }
+
+} // namespace
+
+extern "C" {
+
+const char *dd_last_error(void) { return g_last_error.c_str(); }
+
+const char *dd_status_string(dd_status status) {
+  switch (status) {
+  case DD_OK: return "ok";
+  case DD_ERR_INVALID_ARGUMENT: return "invalid argument";
+  case DD_ERR_UNSUPPORTED: return "unsupported";
+  case DD_ERR_HIP: return "HIP runtime error";
+  case DD_ERR_OUT_OF_MEMORY: return "out of memory";
+  case DD_ERR_CONFIGURATION: return "invalid kernel configuration";
+  case DD_ERR_NOT_READY: return "not ready";
+  case DD_ERR_IO: return "I/O error";
+  }
+  return "unknown status";
+}
+
+uint32_t dd_abi_version(void) { return DD_ABI_VERSION; }
+
+dd_status dd_plan_create(dd_plan **out, const dd_observation *obs_in, uint32_t padding) {
+  if (!out || !obs_in) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  const dd_observation &o = *obs_in;
+  if (o.nr_beams == 0 || o.nr_synthesized_beams == 0) return fail(DD_ERR_INVALID_ARGUMENT, "beam counts must be positive");
+  if (o.nr_channels < 1) return fail(DD_ERR_INVALID_ARGUMENT, "nr_channels must be positive");
+  if (o.nr_subbands < 1 || o.nr_channels % o.nr_subbands != 0)
+    return fail(DD_ERR_INVALID_ARGUMENT, "nr_subbands must divide nr_channels");
+  if (o.nr_samples_per_batch == 0 || o.downsampling == 0 || o.nr_samples_per_batch / o.downsampling == 0)
+    return fail(DD_ERR_INVALID_ARGUMENT, "nr_samples_per_batch / downsampling must be positive");
+  if (!(o.sampling_time > 0.0f)) return fail(DD_ERR_INVALID_ARGUMENT, "sampling_time must be positive");
+  if (o.nr_dms == 0) return fail(DD_ERR_INVALID_ARGUMENT, "nr_dms must be positive");
+  if (!(o.first_dm >= 0.0f) || !(o.dm_step >= 0.0f) || !(o.first_dm_subbanding >= 0.0f) ||
+      !(o.dm_step_subbanding >= 0.0f))
+    return fail(DD_ERR_INVALID_ARGUMENT, "DM ranges must be non-negative");
+  if (padding % 4 != 0) return fail(DD_ERR_INVALID_ARGUMENT, "padding must be a multiple of 4 bytes");
+  if (o.nr_channels > 65535 || o.nr_dms > (1u << 22) || o.nr_dms_subbanding > (1u << 22))
+    return fail(DD_ERR_UNSUPPORTED, "too many channels or DMs");
+
+  dd_plan *p = new dd_plan();
+  p->obs = o;
+  p->padding = padding;
+  p->two_step = o.nr_dms_subbanding > 0;
+  if (p->two_step && o.downsampling != 1) {
+    delete p;
+    return fail(DD_ERR_UNSUPPORTED, "the subband path needs downsampling 1");
+  }
+  (void)hipGetDevice(&p->device);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, p->device) == hipSuccess) p->max_lds = prop.sharedMemPerBlock;
+
+  auto cleanup = [&](dd_status s) {
+    delete p;
+    return s;
+  };
+
+  // Float tables on the device (src/Shifts.cpp:20-42), copied back for sizing.
+  ShiftFormula f{};
+  f.first_freq = o.min_freq;
+  f.freq_step = o.channel_bandwidth;
+  f.max_freq = max_freq(o);
+  f.samples_per_batch = o.nr_samples_per_batch;
+  f.downsampling = o.downsampling;
+  f.sampling_time = o.sampling_time;
+  f.rows = o.nr_channels;
+  f.padded_rows = padto(o.nr_channels, padding / 4);
+  hipError_t e = p->shifts.allocate(f.padded_rows);
+  if (e == hipSuccess) e = launch_build_shifts(f, p->shifts.ptr, nullptr);
+  ShiftFormula f2 = f;
+  f2.first_freq = subband_min_freq(o);
+  f2.freq_step = subband_bandwidth(o);
+  f2.max_freq = subband_max_freq(o);
+  f2.rows = o.nr_subbands;
+  f2.padded_rows = padto(o.nr_subbands, padding / 4);
+  if (e == hipSuccess) e = p->shifts2.allocate(f2.padded_rows);
+  if (e == hipSuccess) e = launch_build_shifts(f2, p->shifts2.ptr, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = p->shifts.download(p->h_shifts);
+  if (e == hipSuccess) e = p->shifts2.download(p->h_shifts2);
+  if (e != hipSuccess) return cleanup(hip_fail(e, "building shift tables"));
+
+  dd_status s = rebuild_integer_tables(p);
+  if (s != DD_OK) return cleanup(s);
+
+  // Sizes (src/DedispersionTest.cpp:129, 145, 146).
+  dd_sizes &z = p->sizes;
+  z.nr_samples = o.nr_samples_per_batch / o.downsampling;
+  z.nr_dispersed_samples = o.nr_samples_per_dispersed_batch
+                               ? o.nr_samples_per_dispersed_batch
+                               : dispersed(o.nr_samples_per_batch, p->h_shifts[0], o.first_dm, o.dm_step, o.nr_dms);
+  if (p->two_step) {
+    z.nr_samples_subbanding = o.nr_samples_per_batch_subbanding
+                                  ? o.nr_samples_per_batch_subbanding
+                                  : dispersed(o.nr_samples_per_batch, p->h_shifts2[0], o.first_dm, o.dm_step, o.nr_dms);
+    z.nr_dispersed_samples_subbanding =
+        o.nr_samples_per_dispersed_batch_subbanding
+            ? o.nr_samples_per_dispersed_batch_subbanding
+            : dispersed(z.nr_samples_subbanding, p->h_shifts[0], o.first_dm_subbanding, o.dm_step_subbanding,
+                        o.nr_dms_subbanding);
+  }
+  const uint32_t C = o.nr_channels;
+  z.in_pitch = padto(z.nr_dispersed_samples, padding);
+  z.out_pitch = padto(z.nr_samples, padding / 4);
+  z.input_elements = (uint64_t)o.nr_beams * C * z.in_pitch;
+  z.output_elements = (uint64_t)o.nr_synthesized_beams * o.nr_dms * z.out_pitch;
+  if (p->two_step) {
+    z.sub_in_pitch = padto(z.nr_dispersed_samples_subbanding, padding);
+    z.sub_out_pitch = padto(z.nr_samples_subbanding, padding / 4);
+    z.sub_input_elements = (uint64_t)o.nr_beams * C * z.sub_in_pitch;
+    z.intermediate_elements = (uint64_t)o.nr_beams * o.nr_dms_subbanding * o.nr_subbands * z.sub_out_pitch;
+    z.sub_output_elements = (uint64_t)o.nr_synthesized_beams * o.nr_dms_subbanding * o.nr_dms * z.out_pitch;
+  }
+
+  // Default tables: no zapped channel, sBeam -> beam sBeam % nrBeams.
+  p->h_zapped.assign(padto(C, padding / 4), 0);
+  s = rebuild_live(p);
+  if (s != DD_OK) return cleanup(s);
+  {
+    const uint32_t mp = padto(C, padding / 4);
+    std::vector<uint32_t> m((size_t)o.nr_synthesized_beams * mp, 0);
+    for (uint32_t b = 0; b < o.nr_synthesized_beams; ++b)
+      for (uint32_t ch = 0; ch < mp; ++ch) m[(size_t)b * mp + ch] = b % o.nr_beams;
+    e = p->map_single.upload(m);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "uploading beam mapping"));
+    const uint32_t mp2 = padto(o.nr_subbands, padding / 4);
+    std::vector<uint32_t> m2((size_t)o.nr_synthesized_beams * mp2, 0);
+    for (uint32_t b = 0; b < o.nr_synthesized_beams; ++b)
+      for (uint32_t sb = 0; sb < mp2; ++sb) m2[(size_t)b * mp2 + sb] = b % o.nr_beams;
+    e = p->map_two.upload(m2);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "uploading beam mapping"));
+  }
+  for (int m = 0; m < 3; ++m) p->conf[m] = default_conf((dd_mode)m);
+  *out = p;
+  return DD_OK;
+}
+
+dd_status dd_plan_destroy(dd_plan *plan) {
+  delete plan;
+  return DD_OK;
+}
+
+dd_status dd_plan_get_sizes(const dd_plan *plan, dd_sizes *sizes) {
+  if (!plan || !sizes) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  *sizes = plan->sizes;
+  return DD_OK;
+}
+
+dd_status dd_plan_set_conf(dd_plan *plan, dd_mode mode, const dd_conf *conf) {
+  if (!plan || !conf || (int)mode < 0 || (int)mode > 2) return fail(DD_ERR_INVALID_ARGUMENT, "invalid argument");
+  if (mode != DD_SINGLE_STEP && !plan->two_step) return fail(DD_ERR_INVALID_ARGUMENT, "plan has no subband path");
+  Geometry g;
+  dd_status s = geometry_of(plan, mode, *conf, g);
+  if (s != DD_OK) return s;
+  plan->conf[mode] = *conf;
+  return DD_OK;
+}
+
+dd_status dd_plan_get_conf(const dd_plan *plan, dd_mode mode, dd_conf *conf) {
+  if (!plan || !conf || (int)mode < 0 || (int)mode > 2) return fail(DD_ERR_INVALID_ARGUMENT, "invalid argument");
+  *conf = plan->conf[mode];
+  return DD_OK;
+}
+
+dd_status dd_plan_set_zapped_channels(dd_plan *plan, const uint32_t *zapped) {
+  if (!plan || !zapped) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  plan->h_zapped.assign(zapped, zapped + plan->h_zapped.size());
+  return rebuild_live(plan);
+}
+
+dd_status dd_plan_set_beam_mapping(dd_plan *plan, const uint32_t *single_step, const uint32_t *step_two) {
+  if (!plan) return fail(DD_ERR_INVALID_ARGUMENT, "null plan");
+  const dd_observation &o = plan->obs;
+  if (single_step) {
+    const size_t n = (size_t)o.nr_synthesized_beams * padto(o.nr_channels, plan->padding / 4);
+    std::vector<uint32_t> m(single_step, single_step + n);
+    for (uint32_t b = 0; b < o.nr_synthesized_beams; ++b)
+      for (uint32_t ch = 0; ch < o.nr_channels; ++ch)
+        if (m[(size_t)b * padto(o.nr_channels, plan->padding / 4) + ch] >= o.nr_beams)
+          return fail(DD_ERR_INVALID_ARGUMENT, "beam mapping entry out of range");
+    DD_HIP(plan->map_single.upload(m));
+  }
+  if (step_two) {
+    const uint32_t mp = padto(o.nr_subbands, plan->padding / 4);
+    const size_t n = (size_t)o.nr_synthesized_beams * mp;
+    std::vector<uint32_t> m(step_two, step_two + n);
+    for (uint32_t b = 0; b < o.nr_synthesized_beams; ++b)
+      for (uint32_t sb = 0; sb < o.nr_subbands; ++sb)
+        if (m[(size_t)b * mp + sb] >= o.nr_beams) return fail(DD_ERR_INVALID_ARGUMENT, "beam mapping entry out of range");
+    DD_HIP(plan->map_two.upload(m));
+  }
+  return DD_OK;
+}
+
+dd_status dd_plan_set_shifts(dd_plan *plan, const float *shifts, const float *shifts_step_two) {
+  if (!plan) return fail(DD_ERR_INVALID_ARGUMENT, "null plan");
+  auto check = [](const float *t, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+      if (!(t[i] >= 0.0f) || !std::isfinite(t[i])) return false;
+    return true;
+  };
+  if (shifts) {
+    if (!check(shifts, plan->obs.nr_channels)) return fail(DD_ERR_INVALID_ARGUMENT, "shift table entries must be finite and non-negative");
+    plan->h_shifts.assign(shifts, shifts + plan->h_shifts.size());
+    DD_HIP(plan->shifts.upload(plan->h_shifts));
+  }
+  if (shifts_step_two) {
+    if (!check(shifts_step_two, plan->obs.nr_subbands)) return fail(DD_ERR_INVALID_ARGUMENT, "shift table entries must be finite and non-negative");
+    plan->h_shifts2.assign(shifts_step_two, shifts_step_two + plan->h_shifts2.size());
+    DD_HIP(plan->shifts2.upload(plan->h_shifts2));
+  }
+  return rebuild_integer_tables(plan);
+}
+
+dd_status dd_plan_get_shifts(const dd_plan *plan, float *shifts, float *shifts_step_two) {
+  if (!plan) return fail(DD_ERR_INVALID_ARGUMENT, "null plan");
+  if (shifts) std::memcpy(shifts, plan->h_shifts.data(), plan->h_shifts.size() * sizeof(float));
+  if (shifts_step_two) std::memcpy(shifts_step_two, plan->h_shifts2.data(), plan->h_shifts2.size() * sizeof(float));
+  return DD_OK;
+}
+
+dd_status dd_plan_get_integer_shifts(const dd_plan *plan, dd_mode mode, uint32_t *table) {
+  if (!plan || !table) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  const ShiftTable *t = mode == DD_SINGLE_STEP ? &plan->ish_single : mode == DD_STEP_ONE ? &plan->ish_one : &plan->ish_two;
+  if (mode != DD_SINGLE_STEP && !plan->two_step) return fail(DD_ERR_INVALID_ARGUMENT, "plan has no subband path");
+  for (uint32_t r = 0; r < t->rows; ++r)
+    for (uint32_t d = 0; d < t->dms; ++d) table[(size_t)r * t->dms + d] = t->at(r, d);
+  return DD_OK;
+}
+
+dd_status dd_dedisperse(dd_plan *plan, const uint8_t *d_input, float *d_output, uint32_t first_sbeam,
+                        uint32_t nr_sbeams, void *stream) {
+  if (!plan || !d_input || !d_output) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  const dd_observation &o = plan->obs;
+  const dd_sizes &z = plan->sizes;
+  if (nr_sbeams == 0) nr_sbeams = o.nr_synthesized_beams - std::min(first_sbeam, o.nr_synthesized_beams);
+  if (nr_sbeams == 0 || first_sbeam + nr_sbeams > o.nr_synthesized_beams)
+    return fail(DD_ERR_INVALID_ARGUMENT, "synthesized beam range out of bounds");
+  if ((uint64_t)z.nr_samples + z.max_shift > z.in_pitch)
+    return fail(DD_ERR_INVALID_ARGUMENT, "dispersed batch shorter than samples + largest shift");
+  const dd_conf &c = plan->conf[DD_SINGLE_STEP];
+  Geometry g;
+  dd_status s = geometry_of(plan, DD_SINGLE_STEP, c, g);
+  if (s != DD_OK) return s;
+  RowSumU8Params k{};
+  k.in = d_input;
+  k.in_bytes = z.input_elements;
+  k.out = d_output;
+  k.ishift = plan->ish_single.dev.ptr;
+  k.dm_pitch = plan->ish_single.pitch;
+  k.live = plan->live_single.ptr;
+  k.live_count = plan->live_single_count.ptr;
+  k.live_pitch = o.nr_channels;
+  k.map = plan->map_single.ptr;
+  k.map_pitch = padto(o.nr_channels, plan->padding / 4);
+  k.first_sbeam = first_sbeam;
+  k.in_row_pitch = z.in_pitch;
+  k.in_beam_pitch = (uint64_t)o.nr_channels * z.in_pitch;
+  k.out_dm_pitch = z.out_pitch;
+  k.out_beam_pitch = (uint64_t)o.nr_dms * z.out_pitch;
+  k.out_group_pitch = 0;
+  k.n_samples = z.nr_samples;
+  k.n_dms = o.nr_dms;
+  k.n_groups = 1;
+  k.unroll = c.unroll;
+  k.win_bytes = g.win;
+  k.nontemporal = 1;
+  k.vec_store = (z.out_pitch % 4 == 0) && aligned16(d_output);
+  const uint32_t ts = 64 * c.items_d0, td = c.threads_d1 * c.items_d1;
+  dim3 grid((z.nr_samples + ts - 1) / ts, (o.nr_dms + td - 1) / td, nr_sbeams);
+  if (grid.y > 65535 || grid.z > 65535) return fail(DD_ERR_UNSUPPORTED, "grid too large");
+  const bool flush = o.nr_channels > 257;
+  DD_HIP(launch_rowsum_u8(k, g, flush, grid, (hipStream_t)stream));
+  return DD_OK;
+}
+
+dd_status dd_subband_step_one(dd_plan *plan, const uint8_t *d_input, float *d_intermediate, void *stream) {
+  if (!plan || !d_input || !d_intermediate) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  if (!plan->two_step) return fail(DD_ERR_INVALID_ARGUMENT, "plan has no subband path");
+  const dd_observation &o = plan->obs;
+  const dd_sizes &z = plan->sizes;
+  if ((uint64_t)z.nr_samples_subbanding + z.max_shift_step_one > z.sub_in_pitch)
+    return fail(DD_ERR_INVALID_ARGUMENT, "dispersed batch shorter than samples + largest shift");
+  const dd_conf &c = plan->conf[DD_STEP_ONE];
+  Geometry g;
+  dd_status s = geometry_of(plan, DD_STEP_ONE, c, g);
+  if (s != DD_OK) return s;
+  RowSumU8Params k{};
+  k.in = d_input;
+  k.in_bytes = z.sub_input_elements;
+  k.out = d_intermediate;
+  k.ishift = plan->ish_one.dev.ptr;
+  k.dm_pitch = plan->ish_one.pitch;
+  k.live = plan->live_sub.ptr;
+  k.live_count = plan->live_sub_count.ptr;
+  k.live_pitch = cps(plan);
+  k.map = nullptr;
+  k.map_pitch = 0;
+  k.first_sbeam = 0;
+  k.in_row_pitch = z.sub_in_pitch;
+  k.in_beam_pitch = (uint64_t)o.nr_channels * z.sub_in_pitch;
+  k.out_dm_pitch = (uint64_t)o.nr_subbands * z.sub_out_pitch;
+  k.out_beam_pitch = (uint64_t)o.nr_dms_subbanding * o.nr_subbands * z.sub_out_pitch;
+  k.out_group_pitch = z.sub_out_pitch;
+  k.n_samples = z.nr_samples_subbanding;
+  k.n_dms = o.nr_dms_subbanding;
+  k.n_groups = o.nr_subbands;
+  k.unroll = c.unroll;
+  k.win_bytes = g.win;
+  k.nontemporal = 0;  // step two re-reads the intermediate
+  k.vec_store = (z.sub_out_pitch % 4 == 0) && aligned16(d_intermediate);
+  const uint32_t ts = 64 * c.items_d0, td = c.threads_d1 * c.items_d1;
+  dim3 grid((z.nr_samples_subbanding + ts - 1) / ts, (o.nr_dms_subbanding + td - 1) / td,
+            o.nr_beams * o.nr_subbands);
+  if (grid.y > 65535 || grid.z > 65535) return fail(DD_ERR_UNSUPPORTED, "grid too large");
+  const bool flush = cps(plan) > 257;
+  DD_HIP(launch_rowsum_u8(k, g, flush, grid, (hipStream_t)stream));
+  return DD_OK;
+}
+
+dd_status dd_subband_step_two(dd_plan *plan, const float *d_intermediate, float *d_output, uint32_t first_sbeam,
+                              uint32_t nr_sbeams, void *stream) {
+  if (!plan || !d_intermediate || !d_output) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  if (!plan->two_step) return fail(DD_ERR_INVALID_ARGUMENT, "plan has no subband path");
+  const dd_observation &o = plan->obs;
+  const dd_sizes &z = plan->sizes;
+  if (nr_sbeams == 0) nr_sbeams = o.nr_synthesized_beams - std::min(first_sbeam, o.nr_synthesized_beams);
+  if (nr_sbeams == 0 || first_sbeam + nr_sbeams > o.nr_synthesized_beams)
+    return fail(DD_ERR_INVALID_ARGUMENT, "synthesized beam range out of bounds");
+  if ((uint64_t)z.nr_samples + z.max_shift_step_two > z.sub_out_pitch)
+    return fail(DD_ERR_INVALID_ARGUMENT, "subbanded batch shorter than samples + largest shift");
+  const dd_conf &c = plan->conf[DD_STEP_TWO];
+  Geometry g;
+  dd_status s = geometry_of(plan, DD_STEP_TWO, c, g);
+  if (s != DD_OK) return s;
+  RowSumF32Params k{};
+  k.in = d_intermediate;
+  k.in_floats = z.intermediate_elements;
+  k.out = d_output;
+  k.ishift = plan->ish_two.dev.ptr;
+  k.dm_pitch = plan->ish_two.pitch;
+  k.map = plan->map_two.ptr;
+  k.map_pitch = padto(o.nr_subbands, plan->padding / 4);
+  k.first_sbeam = first_sbeam;
+  k.n_rows = o.nr_subbands;
+  k.n_coarse = o.nr_dms_subbanding;
+  k.in_row_pitch = z.sub_out_pitch;
+  k.in_coarse_pitch = (uint64_t)o.nr_subbands * z.sub_out_pitch;
+  k.in_beam_pitch = (uint64_t)o.nr_dms_subbanding * o.nr_subbands * z.sub_out_pitch;
+  k.out_dm_pitch = z.out_pitch;
+  k.out_beam_pitch = (uint64_t)o.nr_dms_subbanding * o.nr_dms * z.out_pitch;
+  k.n_samples = z.nr_samples;
+  k.n_dms = o.nr_dms;
+  k.unroll = c.unroll;
+  k.win_floats = g.win;
+  k.nontemporal = 1;
+  const uint32_t ts = 64 * c.items_d0, td = c.threads_d1 * c.items_d1;
+  dim3 grid((z.nr_samples + ts - 1) / ts, (o.nr_dms + td - 1) / td, nr_sbeams * o.nr_dms_subbanding);
+  if (grid.y > 65535 || grid.z > 65535) return fail(DD_ERR_UNSUPPORTED, "grid too large");
+  DD_HIP(launch_rowsum_f32(k, g, grid, (hipStream_t)stream));
+  return DD_OK;
+}
+
+dd_status dd_synth_gaussian_u8(uint8_t *d_data, uint64_t rows, uint64_t cols, uint64_t pitch, float mean,
+                               float std_dev, uint64_t seed, void *stream) {
+  if (!d_data || cols > pitch) return fail(DD_ERR_INVALID_ARGUMENT, "invalid buffer");
+  DD_HIP(launch_synth_gaussian(d_data, rows, cols, pitch, mean, std_dev, seed, false, (hipStream_t)stream));
+  return DD_OK;
+}
+
+dd_status dd_synth_uniform_u8(uint8_t *d_data, uint64_t rows, uint64_t cols, uint64_t pitch, uint64_t seed,
+                              void *stream) {
+  if (!d_data || cols > pitch) return fail(DD_ERR_INVALID_ARGUMENT, "invalid buffer");
+  DD_HIP(launch_synth_gaussian(d_data, rows, cols, pitch, 0.0f, 0.0f, seed, true, (hipStream_t)stream));
+  return DD_OK;
+}
+
+dd_status dd_synth_inject_pulse(dd_plan *plan, uint8_t *d_beam_input, uint64_t pitch, uint32_t row_len,
+                                float dm_value, uint32_t t0, uint32_t width, uint32_t amplitude, void *stream) {
+  if (!plan || !d_beam_input || row_len > pitch) return fail(DD_ERR_INVALID_ARGUMENT, "invalid argument");
+  if (!(dm_value >= 0.0f)) return fail(DD_ERR_INVALID_ARGUMENT, "dm_value must be non-negative");
+  DD_HIP(launch_inject_pulse(d_beam_input, pitch, plan->shifts.ptr, plan->obs.nr_channels, nullptr, dm_value, t0,
+                             width, amplitude, row_len, (hipStream_t)stream));
+  return DD_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,198 @@
+/*
+ * dedispersion_amd.h -- C ABI of the MI355X dedispersion engine.
+ *
+ * The reference (TRASAL/Dedispersion) exposes its device path as OpenCL
+ * source generators whose kernels a caller compiles and launches with
+ * positional arguments:
+ *   single step  (include/Dedispersion.hpp:63, args src/DedispersionTest.cpp:380-385)
+ *   step one     (include/Dedispersion.hpp:64, args src/DedispersionTest.cpp:388-391)
+ *   step two     (include/Dedispersion.hpp:65, args src/DedispersionTest.cpp:393-397)
+ * This header replaces those generators and that launch protocol with a
+ * plan object: the observation and padding fix every size and layout, the
+ * tables (shifts, zapped channels, beam mapping) become device-resident at
+ * plan set-up, and each launch takes device pointers laid out exactly as
+ * the reference's buffers are.
+ *
+ * Conventions: plain C types only; every function returns a dd_status;
+ * device pointers are HIP device pointers; a stream argument is a
+ * hipStream_t passed as void* (NULL = the default stream).  A plan is used
+ * by one host thread at a time; distinct plans may run concurrently.
+ */
+#ifndef DEDISPERSION_AMD_H
+#define DEDISPERSION_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DD_ABI_VERSION 1
+
+typedef enum dd_status {
+  DD_OK = 0,
+  DD_ERR_INVALID_ARGUMENT = 1, /* a size, pointer or table value is invalid */
+  DD_ERR_UNSUPPORTED = 2,      /* valid for the reference, not supported here */
+  DD_ERR_HIP = 3,              /* a HIP runtime call failed */
+  DD_ERR_OUT_OF_MEMORY = 4,
+  DD_ERR_CONFIGURATION = 5,    /* the kernel configuration cannot run */
+  DD_ERR_NOT_READY = 6,        /* a required table has not been set */
+  DD_ERR_IO = 7                /* a file could not be read or written */
+} dd_status;
+
+/* Observation metadata: the AstroData::Observation fields the path reads
+ * (include/Dedispersion.hpp:72-171, src/Shifts.cpp:21-40).  Zero for a
+ * dispersed/subbanding sample count means "derive it with the reference's
+ * sizing rule" (src/DedispersionTest.cpp:129,145,146). */
+typedef struct dd_observation {
+  uint32_t nr_beams;
+  uint32_t nr_synthesized_beams;
+  uint32_t nr_channels;
+  uint32_t nr_subbands;           /* 1 for single-step only plans */
+  float min_freq;                 /* MHz, frequency of channel 0 */
+  float channel_bandwidth;        /* MHz */
+  float sampling_time;            /* s */
+  uint32_t downsampling;          /* >= 1 */
+  uint32_t nr_samples_per_batch;  /* S: output samples of single step / step two */
+  uint32_t nr_dms;                /* D (single step) or fine DMs Df (step two) */
+  float first_dm;
+  float dm_step;
+  uint32_t nr_dms_subbanding;     /* Dc (step one); 0 for single-step plans */
+  float first_dm_subbanding;
+  float dm_step_subbanding;
+  uint32_t nr_samples_per_dispersed_batch;             /* S_disp, 0 = derive */
+  uint32_t nr_samples_per_batch_subbanding;            /* S1, 0 = derive */
+  uint32_t nr_samples_per_dispersed_batch_subbanding;  /* S1_disp, 0 = derive */
+} dd_observation;
+
+/* Kernel configuration, the fields of Dedispersion::DedispersionConf
+ * (include/Dedispersion.hpp:34-54) as they apply to the HIP kernels:
+ *   threads_d0  threads along samples (64, one wavefront)
+ *   threads_d1  wavefronts along DMs per workgroup
+ *   items_d0    samples per thread (4, 8 or 16)
+ *   items_d1    DMs per thread (1, 2, 4 or 8)
+ *   unroll      channels staged per barrier (1..16)
+ *   local       1 = stage rows in LDS, 0 = read rows through the caches
+ *   split_batches must be 0 (unimplemented in the reference too). */
+typedef struct dd_conf {
+  uint32_t threads_d0;
+  uint32_t threads_d1;
+  uint32_t items_d0;
+  uint32_t items_d1;
+  uint32_t unroll;
+  uint32_t local;
+  uint32_t split_batches;
+} dd_conf;
+
+typedef enum dd_mode {
+  DD_SINGLE_STEP = 0,
+  DD_STEP_ONE = 1,
+  DD_STEP_TWO = 2
+} dd_mode;
+
+/* Sizes of the device buffers of a plan, in elements of their type, using
+ * the reference layouts:
+ *   input        uint8 [beams][C][in_pitch]              (include/Dedispersion.hpp:90)
+ *   output       float [sBeams][D][out_pitch]            (include/Dedispersion.hpp:105)
+ *   step-one in  uint8 [beams][C][sub_in_pitch]          (include/Dedispersion.hpp:133)
+ *   intermediate float [beams][Dc][Nsb][sub_out_pitch]   (include/Dedispersion.hpp:148)
+ *   step-two out float [sBeams][Dc*Df][out_pitch]        (include/Dedispersion.hpp:171) */
+typedef struct dd_sizes {
+  uint32_t nr_samples;                 /* S / downsampling */
+  uint32_t nr_dispersed_samples;       /* S_disp */
+  uint32_t nr_samples_subbanding;      /* S1 */
+  uint32_t nr_dispersed_samples_subbanding; /* S1_disp */
+  uint64_t in_pitch;                   /* bytes per channel row, single step */
+  uint64_t out_pitch;                  /* floats per DM row */
+  uint64_t sub_in_pitch;               /* bytes per channel row, step one */
+  uint64_t sub_out_pitch;              /* floats per subband row of the intermediate */
+  uint64_t input_elements;             /* single-step input */
+  uint64_t output_elements;            /* single-step output */
+  uint64_t sub_input_elements;         /* step-one input */
+  uint64_t intermediate_elements;      /* step-one output = step-two input */
+  uint64_t sub_output_elements;        /* step-two output */
+  uint32_t max_shift;                  /* largest integer shift, single step */
+  uint32_t max_shift_step_one;
+  uint32_t max_shift_step_two;
+} dd_sizes;
+
+typedef struct dd_plan dd_plan;
+
+/* Human-readable text of the last error on this thread. */
+const char *dd_last_error(void);
+const char *dd_status_string(dd_status status);
+uint32_t dd_abi_version(void);
+
+/* Creates a plan on the current HIP device.  padding is the reference's
+ * `padding` argument in bytes (a row of T elements is padded to a multiple
+ * of padding/sizeof(T); 0 = unpadded).  The shift tables are built on the
+ * device from the observation with the arithmetic of src/Shifts.cpp:20-42;
+ * all channels start live and every synthesized beam maps to input beam
+ * (sBeam % nr_beams). */
+dd_status dd_plan_create(dd_plan **plan, const dd_observation *obs, uint32_t padding);
+dd_status dd_plan_destroy(dd_plan *plan);
+dd_status dd_plan_get_sizes(const dd_plan *plan, dd_sizes *sizes);
+
+/* Kernel configuration per mode (defaults are chosen at creation). */
+dd_status dd_plan_set_conf(dd_plan *plan, dd_mode mode, const dd_conf *conf);
+dd_status dd_plan_get_conf(const dd_plan *plan, dd_mode mode, dd_conf *conf);
+
+/* Zapped channels, the reference's uint32 flags [pad(C, padding/4)]
+ * (src/DedispersionTest.cpp:120); non-zero = zapped.  Host pointer.  Packed
+ * into a device bitmask. */
+dd_status dd_plan_set_zapped_channels(dd_plan *plan, const uint32_t *zapped);
+/* Beam mappings (host pointers, NULL = keep): single step
+ * uint32 [sBeams][pad(C, padding/4)], step two uint32 [sBeams][pad(Nsb, padding/4)]
+ * (src/DedispersionTest.cpp:121-122). */
+dd_status dd_plan_set_beam_mapping(dd_plan *plan, const uint32_t *single_step,
+                                   const uint32_t *step_two);
+/* Replaces the device shift tables with caller tables (host pointers, NULL =
+ * keep): shifts [pad(C, padding/4)] as getShifts returns, step-two shifts
+ * [pad(Nsb, padding/4)] as getShiftsStepTwo returns. */
+dd_status dd_plan_set_shifts(dd_plan *plan, const float *shifts, const float *shifts_step_two);
+/* Copies the device tables back (host pointers, NULL = skip).  Integer
+ * tables are [rows][dms] with the reference's integer shift per (row, DM):
+ * single [C][D], step one [C][Dc] (relative to the subband's top channel),
+ * step two [Nsb][Df]. */
+dd_status dd_plan_get_shifts(const dd_plan *plan, float *shifts, float *shifts_step_two);
+dd_status dd_plan_get_integer_shifts(const dd_plan *plan, dd_mode mode, uint32_t *table);
+
+/* Single-step dedispersion of sBeams [first, first + n) (n = 0: all).
+ * Replaces the `dedispersion` kernel (include/Dedispersion.hpp:203-540). */
+dd_status dd_dedisperse(dd_plan *plan, const uint8_t *d_input, float *d_output,
+                        uint32_t first_synthesized_beam, uint32_t nr_synthesized_beams,
+                        void *stream);
+/* Step one: channels into subbands at the coarse DMs, with the shift
+ * relative to each subband's top channel (include/Dedispersion.hpp:124).
+ * Replaces `dedispersionStepOne` (include/Dedispersion.hpp:543-882). */
+dd_status dd_subband_step_one(dd_plan *plan, const uint8_t *d_input, float *d_intermediate,
+                              void *stream);
+/* Step two: subbands at the fine DMs.  Replaces `dedispersionStepTwo`
+ * (include/Dedispersion.hpp:884-1122). */
+dd_status dd_subband_step_two(dd_plan *plan, const float *d_intermediate, float *d_output,
+                              uint32_t first_synthesized_beam, uint32_t nr_synthesized_beams,
+                              void *stream);
+
+/* ---- synthetic inputs (seeded, reproducible) ---- */
+
+/* Fills rows x cols uint8 values (row pitch `pitch` bytes) with
+ * round(mean + std * N(0,1)) clipped to [0, 255]; the normal deviates come
+ * from a counter-based generator keyed by (seed, row, col). */
+dd_status dd_synth_gaussian_u8(uint8_t *d_data, uint64_t rows, uint64_t cols, uint64_t pitch,
+                               float mean, float std_dev, uint64_t seed, void *stream);
+/* Fills uint8 values uniform in [0, 255] keyed by (seed, row, col). */
+dd_status dd_synth_uniform_u8(uint8_t *d_data, uint64_t rows, uint64_t cols, uint64_t pitch,
+                              uint64_t seed, void *stream);
+/* Adds `amplitude` (clipped at 255) to a dispersed boxcar: for every channel
+ * c of one beam ([C][pitch] bytes), samples [t0 + shift(c), + width) below
+ * row_len, where shift(c) = (unsigned)(dm_value * shifts[c]) with the plan's
+ * single-step table (include/Dedispersion.hpp:81). */
+dd_status dd_synth_inject_pulse(dd_plan *plan, uint8_t *d_beam_input, uint64_t pitch,
+                                uint32_t row_len, float dm_value, uint32_t t0, uint32_t width,
+                                uint32_t amplitude, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,269 @@
+"""Parity of the HIP kernels with the CPU oracle, through the C ABI.
+
+uint8 paths (single step, step one) must be bit-exact: every partial sum is an
+integer below 2^24.  Step two must be bit-exact too, for any float input,
+because it adds the subbands in the reference's order.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle
+from dedispersion_amd import DD_SINGLE_STEP, DD_STEP_ONE, DD_STEP_TWO, Conf, DDError, Plan, workloads
+from tests.conftest import host_threads
+from tests.gpu_helpers import compare_rows, make_input, single_step, stream_of, zapped_flags
+from tests.test_oracle import int_shifts
+
+pytestmark = pytest.mark.gpu
+
+
+# ------------------------------------------------------------------ tables
+
+
+@pytest.mark.parametrize("make", [workloads.plain_128, workloads.arts_single, workloads.arts_two_step])
+def test_device_tables_bit_exact(torch_cuda, make):
+    w = make()
+    plan = Plan(w.obs, 128)
+    orc = oracle.Oracle(w.obs, 128)
+    s, s2 = plan.get_shifts()
+    np.testing.assert_array_equal(s.view(np.uint32), orc.shifts.view(np.uint32))
+    np.testing.assert_array_equal(s2.view(np.uint32), orc.shifts2.view(np.uint32))
+    C, Nsb = w.obs.nr_channels, w.obs.nr_subbands
+    np.testing.assert_array_equal(plan.get_integer_shifts(DD_SINGLE_STEP),
+                                  int_shifts(orc.shifts[:C], w.obs.first_dm, w.obs.dm_step, w.obs.nr_dms))
+    z = plan.sizes
+    assert z.nr_dispersed_samples == orc.o.nr_samples_per_dispersed_batch
+    assert z.in_pitch == orc.in_pitch and z.out_pitch == orc.out_pitch
+    if w.obs.nr_dms_subbanding:
+        cps = C // Nsb
+        rel = (orc.shifts[:C] - np.repeat(orc.shifts[(np.arange(Nsb) + 1) * cps - 1], cps)).astype(np.float32)
+        np.testing.assert_array_equal(
+            plan.get_integer_shifts(DD_STEP_ONE),
+            int_shifts(rel, w.obs.first_dm_subbanding, w.obs.dm_step_subbanding, w.obs.nr_dms_subbanding))
+        np.testing.assert_array_equal(plan.get_integer_shifts(DD_STEP_TWO),
+                                      int_shifts(orc.shifts2[:Nsb], w.obs.first_dm, w.obs.dm_step, w.obs.nr_dms))
+        assert z.nr_samples_subbanding == orc.o.nr_samples_per_batch_sub
+        assert z.nr_dispersed_samples_subbanding == orc.o.nr_samples_per_dispersed_batch_sub
+        assert z.sub_in_pitch == orc.sub_in_pitch and z.sub_out_pitch == orc.sub_out_pitch
+
+
+# -------------------------------------------------------------- single step
+
+
+def test_single_step_plain_128(torch_cuda):
+    """128 channels, 4096 samples, 128 DMs, channels 17 and 90 zapped."""
+    w = workloads.plain_128()
+    plan, inp, out, orc, flags = single_step(torch_cuda, w)
+    want = orc.dedispersion(inp, flags)
+    compare_rows(out.reshape(want.shape), want, w.obs.nr_samples_per_batch, "plain-128")
+
+
+SWEEP_CONFS = [
+    Conf(64, t1, i0, i1, u, loc)
+    for (t1, i0, i1, u, loc) in itertools.product((1, 2, 4, 8), (4, 8, 16), (1, 2, 4, 8), (1, 3, 8), (1, 0))
+    if t1 * i1 <= 64
+]
+
+
+@pytest.fixture(scope="module")
+def ragged_case(torch_cuda):
+    """Odd sizes everywhere: 3 beams, 4 synthesized beams with a mixed
+    mapping, 300 channels (> 257, so the 16-bit accumulators flush), 1000
+    samples, 37 DMs, unpadded rows (padding 0)."""
+    w = workloads.tiny(channels=300, samples=1000, dms=37, step=0.7, beams=3, sbeams=4, min_freq=1300.0,
+                       bandwidth=0.5, zapped=(0, 5, 150, 299))
+    plan = Plan(w.obs, 0)
+    flags = zapped_flags(plan, w.zapped)
+    plan.set_zapped_channels(flags)
+    mapping = (np.arange(4)[:, None] % 3 + np.zeros((1, 300), dtype=int)).astype(np.uint32)
+    mapping[3, ::2] = 2
+    plan.set_beam_mapping(single=mapping)
+    z = plan.sizes
+    d_in = make_input(torch_cuda, plan, w, z.in_pitch, z.nr_dispersed_samples, 11)
+    orc = oracle.Oracle(w.obs, 0, threads=host_threads())
+    want = orc.dedispersion(d_in.cpu().numpy(), flags, mapping)
+    return w, plan, d_in, want
+
+
+@pytest.mark.parametrize("conf", SWEEP_CONFS, ids=lambda c: f"t{c.threads_d1}i{c.items_d0}x{c.items_d1}u{c.unroll}l{c.local}")
+def test_single_step_conf_sweep(torch_cuda, ragged_case, conf):
+    w, plan, d_in, want = ragged_case
+    plan.set_conf(DD_SINGLE_STEP, conf)
+    z = plan.sizes
+    d_out = torch_cuda.full((z.output_elements,), -1.0, dtype=torch_cuda.float32, device="cuda")
+    plan.dedisperse(d_in.data_ptr(), d_out.data_ptr(), stream=stream_of(torch_cuda))
+    got = d_out.cpu().numpy().reshape(want.shape)
+    compare_rows(got, want, w.obs.nr_samples_per_batch, f"conf {conf}")
+    # padding of the output rows is never written
+    assert np.all(got[..., w.obs.nr_samples_per_batch:] == -1.0)
+
+
+def test_single_step_beam_subrange(torch_cuda, ragged_case):
+    """firstSynthesizedBeam offsets the beam mapping; the output is relative
+    to the launch (include/Dedispersion.hpp:228, 267)."""
+    w, plan, d_in, want = ragged_case
+    plan.set_conf(DD_SINGLE_STEP, Conf())
+    z = plan.sizes
+    per_beam = w.obs.nr_dms * z.out_pitch
+    d_out = torch_cuda.zeros(2 * per_beam, dtype=torch_cuda.float32, device="cuda")
+    plan.dedisperse(d_in.data_ptr(), d_out.data_ptr(), first_sbeam=2, nr_sbeams=2, stream=stream_of(torch_cuda))
+    got = d_out.cpu().numpy().reshape(2, w.obs.nr_dms, z.out_pitch)
+    compare_rows(got, want[2:4], w.obs.nr_samples_per_batch, "beams 2..3")
+
+
+def test_single_step_arts(torch_cuda):
+    """ARTS-sized single step: 1536 channels, 2048 DMs, 24576 samples, 77
+    zapped channels, Gaussian noise and 8 dispersed pulses.  The oracle checks
+    96 DM rows across the range; the DM-0 row and the pulses are checked on
+    every sample."""
+    w = workloads.arts_single()
+    plan, inp, out, orc, flags = single_step(torch_cuda, w)
+    D, S = w.obs.nr_dms, w.obs.nr_samples_per_batch
+    out = out.reshape(1, D, plan.sizes.out_pitch)
+    for r0 in (0, 1000, D - 32):
+        want = orc.dedispersion(inp, flags, rows=(r0, r0 + 32))
+        compare_rows(out[:, r0:r0 + 32], want[:, r0:r0 + 32], S, f"DM rows {r0}..{r0 + 32}")
+    # DM 0: plain sum of the live channels
+    x = inp.reshape(w.obs.nr_channels, plan.sizes.in_pitch)[:, :S].astype(np.float64)
+    x[np.asarray(w.zapped)] = 0
+    np.testing.assert_array_equal(out[0, 0, :S], x.sum(axis=0))
+    # each pulse is recovered at its DM: the boxcar region stands out
+    live = w.obs.nr_channels - len(w.zapped)
+    for p in w.pulses:
+        row = out[0, p.dm_index, :S]
+        peak = row[p.t0:p.t0 + p.width].mean()
+        assert peak > np.median(row) + 0.25 * live * p.amplitude, (p, peak, np.median(row))
+
+
+# --------------------------------------------------------------- two step
+
+
+def two_step_case(torch, w, padding=128, conf1=None, conf2=None, seed=None):
+    plan = Plan(w.obs, padding)
+    flags = zapped_flags(plan, w.zapped)
+    plan.set_zapped_channels(flags)
+    if conf1 is not None:
+        plan.set_conf(DD_STEP_ONE, conf1)
+    if conf2 is not None:
+        plan.set_conf(DD_STEP_TWO, conf2)
+    z = plan.sizes
+    d_in = make_input(torch, plan, w, z.sub_in_pitch, z.nr_dispersed_samples_subbanding,
+                      w.seed if seed is None else seed)
+    d_mid = torch.full((z.intermediate_elements,), -1.0, dtype=torch.float32, device="cuda")
+    d_out = torch.full((z.sub_output_elements,), -1.0, dtype=torch.float32, device="cuda")
+    s = stream_of(torch)
+    plan.step_one(d_in.data_ptr(), d_mid.data_ptr(), stream=s)
+    plan.step_two(d_mid.data_ptr(), d_out.data_ptr(), stream=s)
+    torch.cuda.synchronize()
+    orc = oracle.Oracle(w.obs, padding, threads=host_threads())
+    return plan, flags, d_in.cpu().numpy(), d_mid.cpu().numpy(), d_out.cpu().numpy(), orc
+
+
+STEP_ONE_CONFS = [
+    Conf(64, t1, i0, i1, u, loc)
+    for (t1, i0, i1, u, loc) in itertools.product((1, 4), (4, 8, 16), (1, 2, 8), (1, 5), (1, 0))
+]
+
+
+@pytest.fixture(scope="module")
+def tiny_two_step(torch_cuda):
+    w = workloads.tiny(channels=96, samples=700, dms=12, step=0.9, subbands=8, coarse=9, coarse_step=4.5, beams=2,
+                       sbeams=3, min_freq=1300.0, bandwidth=1.0, zapped=(0, 11, 12, 50, 95))
+    plan = Plan(w.obs, 0)
+    flags = zapped_flags(plan, w.zapped)
+    plan.set_zapped_channels(flags)
+    z = plan.sizes
+    d_in = make_input(torch_cuda, plan, w, z.sub_in_pitch, z.nr_dispersed_samples_subbanding, 21)
+    orc = oracle.Oracle(w.obs, 0, threads=host_threads())
+    want_mid = orc.step_one(d_in.cpu().numpy(), flags)
+    return w, plan, d_in, orc, want_mid
+
+
+@pytest.mark.parametrize("conf", STEP_ONE_CONFS, ids=lambda c: f"t{c.threads_d1}i{c.items_d0}x{c.items_d1}u{c.unroll}l{c.local}")
+def test_step_one_conf_sweep(torch_cuda, tiny_two_step, conf):
+    w, plan, d_in, orc, want = tiny_two_step
+    plan.set_conf(DD_STEP_ONE, conf)
+    z = plan.sizes
+    d_mid = torch_cuda.full((z.intermediate_elements,), -1.0, dtype=torch_cuda.float32, device="cuda")
+    plan.step_one(d_in.data_ptr(), d_mid.data_ptr(), stream=stream_of(torch_cuda))
+    got = d_mid.cpu().numpy().reshape(want.shape)
+    compare_rows(got, want, z.nr_samples_subbanding, f"step one {conf}")
+
+
+STEP_TWO_CONFS = [
+    Conf(64, t1, i0, i1, u, loc)
+    for (t1, i0, i1, u, loc) in itertools.product((1, 2, 4), (4, 8, 16), (1, 4, 8), (1, 3), (1, 0))
+]
+
+
+@pytest.fixture(scope="module")
+def tiny_step_two_floats(torch_cuda, tiny_two_step):
+    """General (non-integer) floats as the step-two input, mixed mapping."""
+    w, plan, _, orc, _ = tiny_two_step
+    z = plan.sizes
+    rng = np.random.default_rng(7)
+    mid = (rng.standard_normal(z.intermediate_elements) * 300.0).astype(np.float32)
+    mapping = orc.default_map(8)
+    mapping[1, :8] = [1, 0, 1, 0, 0, 1, 1, 0]
+    mapping[2, :8] = 0
+    plan.set_beam_mapping(step_two=mapping)
+    want = orc.step_two(mid, mapping)
+    d_mid = torch_cuda.from_numpy(mid).cuda()
+    return w, plan, d_mid, want
+
+
+@pytest.mark.parametrize("conf", STEP_TWO_CONFS, ids=lambda c: f"t{c.threads_d1}i{c.items_d0}x{c.items_d1}u{c.unroll}l{c.local}")
+def test_step_two_conf_sweep_general_floats(torch_cuda, tiny_step_two_floats, conf):
+    w, plan, d_mid, want = tiny_step_two_floats
+    plan.set_conf(DD_STEP_TWO, conf)
+    z = plan.sizes
+    d_out = torch_cuda.full((z.sub_output_elements,), -1.0, dtype=torch_cuda.float32, device="cuda")
+    plan.step_two(d_mid.data_ptr(), d_out.data_ptr(), stream=stream_of(torch_cuda))
+    got = d_out.cpu().numpy().reshape(want.shape)
+    compare_rows(got, want, w.obs.nr_samples_per_batch, f"step two {conf}")
+
+
+def test_two_step_arts(torch_cuda):
+    """ARTS two-step: 32 subbands of 48 channels, 64 coarse DMs of 6.4, 32
+    fine DMs of 0.2.  Step one is checked against the oracle on 12 coarse DMs
+    (all subbands), step two on the GPU intermediate for 6 coarse DMs."""
+    w = workloads.arts_two_step()
+    plan, flags, inp, mid, out, orc = two_step_case(torch_cuda, w)
+    z = plan.sizes
+    o = w.obs
+    mid = mid.reshape(1, o.nr_dms_subbanding, o.nr_subbands, z.sub_out_pitch)
+    for c0 in (0, 30, 58):
+        want = orc.step_one(inp, flags, rows=(c0, c0 + 4))
+        compare_rows(mid[:, c0:c0 + 4], want[:, c0:c0 + 4], z.nr_samples_subbanding, f"step one coarse {c0}")
+    out = out.reshape(1, o.nr_dms_subbanding * o.nr_dms, z.out_pitch)
+    for c0 in (0, 33, 62):
+        r0, r1 = c0 * o.nr_dms, (c0 + 2) * o.nr_dms
+        want = orc.step_two(mid.ravel(), rows=(r0, r1))
+        compare_rows(out[:, r0:r1], want[:, r0:r1], o.nr_samples_per_batch, f"step two coarse {c0}")
+    # intermediate padding is never written; step two output rows are complete
+    assert np.all(mid[..., z.nr_samples_subbanding:] == -1.0)
+    assert np.all(out[..., : o.nr_samples_per_batch] >= 0)
+
+
+# ------------------------------------------------------------ error paths
+
+
+def test_invalid_configurations_are_rejected(torch_cuda):
+    w = workloads.plain_128()
+    plan = Plan(w.obs, 128)
+    for bad in (Conf(threads_d0=32), Conf(threads_d1=0), Conf(threads_d1=9), Conf(items_d0=5), Conf(items_d1=3),
+                Conf(unroll=0), Conf(unroll=17), Conf(split_batches=1), Conf(threads_d1=8, items_d1=16)):
+        with pytest.raises(DDError):
+            plan.set_conf(DD_SINGLE_STEP, bad)
+    with pytest.raises(DDError):
+        plan.set_conf(DD_STEP_ONE, Conf())  # no subband path in this plan
+    with pytest.raises(DDError):
+        plan.dedisperse(0, 0)
+    # a dispersed batch shorter than samples + largest shift is refused
+    o = workloads.plain_128().obs
+    o.nr_samples_per_dispersed_batch = o.nr_samples_per_batch + 10
+    short = Plan(o, 128)
+    buf = torch_cuda.zeros(16, dtype=torch_cuda.float32, device="cuda")
+    with pytest.raises(DDError):
+        short.dedisperse(buf.data_ptr(), buf.data_ptr())

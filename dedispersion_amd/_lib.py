@@ -104,6 +104,11 @@ PROTOTYPES = {
     "dd_dedisperse": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_void_p]),
     "dd_subband_step_one": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "dd_subband_step_two": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_void_p]),
+    "dd_tune_time": (
+        c_int,
+        [c_void_p, c_int, POINTER(DDConf), c_void_p, c_void_p, c_uint32, c_uint32, c_void_p, POINTER(c_float),
+         POINTER(c_float)],
+    ),
     "dd_synth_gaussian_u8": (c_int, [c_void_p, c_uint64, c_uint64, c_uint64, c_float, c_float, c_uint64, c_void_p]),
     "dd_synth_uniform_u8": (c_int, [c_void_p, c_uint64, c_uint64, c_uint64, c_uint64, c_void_p]),
     "dd_synth_inject_pulse": (

@@ -40,6 +40,10 @@ struct RowSumU8Params {
   uint32_t win_bytes;         // LDS window per row, bytes (multiple of 16)
   uint32_t nontemporal;       // stream the output past the caches
   uint32_t vec_store;         // output rows 16-byte aligned
+  // register-window kernels: shifts computed from the float table
+  const float *table;         // float delay per row (samples per unit DM)
+  float first_dm, dm_step;
+  uint32_t n_tiles_x, n_tiles_y, n_tiles;  // 1-D grid decomposition
 };
 
 // Shifted row sum over float rows: step two (include/Dedispersion.hpp:155-176).
@@ -66,6 +70,10 @@ struct RowSumF32Params {
   uint32_t unroll;
   uint32_t win_floats;        // LDS window per row, floats (multiple of 4)
   uint32_t nontemporal;
+  uint32_t vec_store;         // output rows 16-byte aligned
+  const float *table;         // float delay per subband
+  float first_dm, dm_step;
+  uint32_t n_tiles_x, n_tiles_y, n_tiles;
 };
 
 // Geometry of one launch.
@@ -76,6 +84,8 @@ struct Geometry {
   uint32_t local;      // LDS staging
   uint32_t win;        // LDS window per row (bytes for uint8 rows, floats for float rows)
   uint32_t lds_bytes;  // dynamic LDS per workgroup
+  uint32_t window;     // 1 = register-window kernel
+  uint32_t span;       // register-window capacity (largest per-wavefront shift spread)
 };
 
 // Launchers (kernels_u8.hip / kernels_f32.hip).  Return hipSuccess or the
@@ -86,6 +96,23 @@ hipError_t launch_rowsum_f32(const RowSumF32Params &p, const Geometry &g, dim3 g
                              hipStream_t stream);
 bool rowsum_u8_supported(uint32_t items_d0, uint32_t items_d1);
 bool rowsum_f32_supported(uint32_t items_d0, uint32_t items_d1);
+// Register-window kernels (kernels_u8_win.hip / kernels_f32_win.hip).
+hipError_t launch_rowsum_u8_win(const RowSumU8Params &p, const Geometry &g, bool flush, hipStream_t stream);
+hipError_t launch_rowsum_f32_win(const RowSumF32Params &p, const Geometry &g, hipStream_t stream);
+// Smallest supported register-window span >= need, or 0.
+uint32_t u8_win_span(uint32_t need);
+uint32_t f32_win_span(uint32_t need);
+// Geometry constants of the register-window kernels.
+constexpr uint32_t kWinSlotsPerWave = 2;   // rows staged per wavefront per group
+constexpr uint32_t kU8WinChunks = 2;       // 16-byte staging loads per lane per row (window <= 2 KiB)
+inline uint32_t u8_win_reads(uint32_t run, uint32_t span) {  // bytes of a lane window
+  const uint32_t emax = run - 1 + span, nwd = emax / 4 + run / 4 + 1;
+  return ((nwd * 4 + run - 1) / run) * run;
+}
+inline uint32_t f32_win_chunks(uint32_t run) { return run / 4 + 1; }  // 16-byte staging loads per lane per row
+inline uint32_t f32_win_reads(uint32_t run, uint32_t span) {  // floats of a lane window
+  return ((3 + span + run + 3) / 4) * 4;
+}
 
 // Table kernels (tables.hip).
 struct ShiftFormula {

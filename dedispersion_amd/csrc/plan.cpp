@@ -219,26 +219,57 @@ dd_status geometry_of(const dd_plan *p, dd_mode mode, const dd_conf &c, Geometry
   g.threads_d1 = c.threads_d1;
   g.items_d0 = c.items_d0;
   g.items_d1 = c.items_d1;
+  if (c.local > 2) return fail(DD_ERR_CONFIGURATION, "local must be 0, 1 or 2");
   g.local = c.local ? 1 : 0;
   g.win = 0;
   g.lds_bytes = 0;
+  g.window = 0;
+  g.span = 0;
   const uint32_t ts = 64 * c.items_d0;
   if (mode == DD_STEP_TWO) {
     if (!rowsum_f32_supported(c.items_d0, c.items_d1))
       return fail(DD_ERR_CONFIGURATION, "items_d0 must be 4, 8 or 16 and items_d1 1, 2, 4 or 8");
     if (g.local) {
-      // lane reads floats [spread + lane + 64k] for k < items_d0
-      g.win = roundup(ts + p->ish_two.max_spread(tile), 4);
-      g.lds_bytes = c.unroll * g.win * 4;
+      const uint32_t tile_spread = p->ish_two.max_spread(tile);
+      if (c.local == 1 && (c.items_d0 == 4 || c.items_d0 == 16) && c.unroll <= kWinSlotsPerWave * c.threads_d1) {
+        const uint32_t span = f32_win_span(p->ish_two.max_spread(c.items_d1));
+        const uint32_t wf = roundup(ts + tile_spread + f32_win_reads(c.items_d0, span), 16);
+        if (span && wf / 4 <= 64 * f32_win_chunks(c.items_d0)) {
+          g.window = 1;
+          g.span = span;
+          g.win = wf;
+          const uint32_t wp = c.items_d0 == 16 ? wf + wf / 4 : wf;
+          g.lds_bytes = 4 * (roundup(2 * p->obs.nr_subbands, 4) + 2 * c.unroll * wp + 2 * c.unroll * tile);
+        }
+      }
+      if (!g.window) {
+        // lane reads floats [spread + lane + 64k] for k < items_d0
+        g.win = roundup(ts + tile_spread, 4);
+        g.lds_bytes = c.unroll * g.win * 4;
+      }
     }
   } else {
     if (!rowsum_u8_supported(c.items_d0, c.items_d1))
       return fail(DD_ERR_CONFIGURATION, "items_d0 must be 4, 8 or 16 and items_d1 1, 2, 4 or 8");
     if (g.local) {
-      // window start misalignment (< 4) + spread + the dword after each group
       const ShiftTable &t = mode == DD_SINGLE_STEP ? p->ish_single : p->ish_one;
-      g.win = roundup(ts + t.max_spread(tile) + 8, 16);
-      g.lds_bytes = c.unroll * g.win;
+      const uint32_t tile_spread = t.max_spread(tile);
+      if (c.local == 1 && c.unroll <= kWinSlotsPerWave * c.threads_d1) {
+        const uint32_t span = u8_win_span(t.max_spread(c.items_d1));
+        const uint32_t win = roundup(ts - c.items_d0 + tile_spread + 3 + u8_win_reads(c.items_d0, span), 16);
+        if (span && win <= 16 * 64 * kU8WinChunks) {
+          const uint32_t live_pitch = mode == DD_SINGLE_STEP ? p->obs.nr_channels : cps(p);
+          g.window = 1;
+          g.span = span;
+          g.win = win;
+          g.lds_bytes = 4 * (roundup(2 * live_pitch, 4) + 2 * c.unroll * (win / 4) + 2 * c.unroll * tile);
+        }
+      }
+      if (!g.window) {
+        // window start misalignment (< 4) + spread + the dword after each group
+        g.win = roundup(ts + tile_spread + 8, 16);
+        g.lds_bytes = c.unroll * g.win;
+      }
     }
   }
   if (g.lds_bytes > p->max_lds)
@@ -286,8 +317,8 @@ dd_status dd_plan_create(dd_plan **out, const dd_observation *obs_in, uint32_t p
       !(o.dm_step_subbanding >= 0.0f))
     return fail(DD_ERR_INVALID_ARGUMENT, "DM ranges must be non-negative");
   if (padding % 4 != 0) return fail(DD_ERR_INVALID_ARGUMENT, "padding must be a multiple of 4 bytes");
-  if (o.nr_channels > 65535 || o.nr_dms > (1u << 22) || o.nr_dms_subbanding > (1u << 22))
-    return fail(DD_ERR_UNSUPPORTED, "too many channels or DMs");
+  if (o.nr_channels > 65535 || o.nr_beams > 65535 || o.nr_dms > (1u << 22) || o.nr_dms_subbanding > (1u << 22))
+    return fail(DD_ERR_UNSUPPORTED, "too many channels, beams or DMs");
 
   dd_plan *p = new dd_plan();
   p->obs = o;
@@ -517,10 +548,20 @@ dd_status dd_dedisperse(dd_plan *plan, const uint8_t *d_input, float *d_output, 
   k.win_bytes = g.win;
   k.nontemporal = 1;
   k.vec_store = (z.out_pitch % 4 == 0) && aligned16(d_output);
+  k.table = plan->shifts.ptr;
+  k.first_dm = o.first_dm;
+  k.dm_step = o.dm_step;
   const uint32_t ts = 64 * c.items_d0, td = c.threads_d1 * c.items_d1;
   dim3 grid((z.nr_samples + ts - 1) / ts, (o.nr_dms + td - 1) / td, nr_sbeams);
-  if (grid.y > 65535 || grid.z > 65535) return fail(DD_ERR_UNSUPPORTED, "grid too large");
   const bool flush = o.nr_channels > 257;
+  if (g.window) {
+    k.n_tiles_x = grid.x;
+    k.n_tiles_y = grid.y;
+    k.n_tiles = grid.x * grid.y * grid.z;
+    DD_HIP(launch_rowsum_u8_win(k, g, flush, (hipStream_t)stream));
+    return DD_OK;
+  }
+  if (grid.y > 65535 || grid.z > 65535) return fail(DD_ERR_UNSUPPORTED, "grid too large");
   DD_HIP(launch_rowsum_u8(k, g, flush, grid, (hipStream_t)stream));
   return DD_OK;
 }
@@ -560,11 +601,21 @@ dd_status dd_subband_step_one(dd_plan *plan, const uint8_t *d_input, float *d_in
   k.win_bytes = g.win;
   k.nontemporal = 0;  // step two re-reads the intermediate
   k.vec_store = (z.sub_out_pitch % 4 == 0) && aligned16(d_intermediate);
+  k.table = plan->relative.ptr;
+  k.first_dm = o.first_dm_subbanding;
+  k.dm_step = o.dm_step_subbanding;
   const uint32_t ts = 64 * c.items_d0, td = c.threads_d1 * c.items_d1;
   dim3 grid((z.nr_samples_subbanding + ts - 1) / ts, (o.nr_dms_subbanding + td - 1) / td,
             o.nr_beams * o.nr_subbands);
-  if (grid.y > 65535 || grid.z > 65535) return fail(DD_ERR_UNSUPPORTED, "grid too large");
   const bool flush = cps(plan) > 257;
+  if (g.window) {
+    k.n_tiles_x = grid.x;
+    k.n_tiles_y = grid.y;
+    k.n_tiles = grid.x * grid.y * grid.z;
+    DD_HIP(launch_rowsum_u8_win(k, g, flush, (hipStream_t)stream));
+    return DD_OK;
+  }
+  if (grid.y > 65535 || grid.z > 65535) return fail(DD_ERR_UNSUPPORTED, "grid too large");
   DD_HIP(launch_rowsum_u8(k, g, flush, grid, (hipStream_t)stream));
   return DD_OK;
 }
@@ -605,10 +656,69 @@ dd_status dd_subband_step_two(dd_plan *plan, const float *d_intermediate, float 
   k.unroll = c.unroll;
   k.win_floats = g.win;
   k.nontemporal = 1;
+  k.vec_store = (z.out_pitch % 4 == 0) && aligned16(d_output);
+  k.table = plan->shifts2.ptr;
+  k.first_dm = o.first_dm;
+  k.dm_step = o.dm_step;
   const uint32_t ts = 64 * c.items_d0, td = c.threads_d1 * c.items_d1;
   dim3 grid((z.nr_samples + ts - 1) / ts, (o.nr_dms + td - 1) / td, nr_sbeams * o.nr_dms_subbanding);
+  if (g.window) {
+    k.n_tiles_x = grid.x;
+    k.n_tiles_y = grid.y;
+    k.n_tiles = grid.x * grid.y * grid.z;
+    DD_HIP(launch_rowsum_f32_win(k, g, (hipStream_t)stream));
+    return DD_OK;
+  }
   if (grid.y > 65535 || grid.z > 65535) return fail(DD_ERR_UNSUPPORTED, "grid too large");
   DD_HIP(launch_rowsum_f32(k, g, grid, (hipStream_t)stream));
+  return DD_OK;
+}
+
+dd_status dd_tune_time(dd_plan *plan, dd_mode mode, const dd_conf *conf, const void *d_input, void *d_output,
+                       uint32_t warmup, uint32_t iterations, void *stream, float *mean_ms, float *std_ms) {
+  if (!plan || !conf || !mean_ms || !std_ms || iterations == 0 || (int)mode < 0 || (int)mode > 2)
+    return fail(DD_ERR_INVALID_ARGUMENT, "invalid argument");
+  if (mode != DD_SINGLE_STEP && !plan->two_step) return fail(DD_ERR_INVALID_ARGUMENT, "plan has no subband path");
+  Geometry g;
+  dd_status s = geometry_of(plan, mode, *conf, g);
+  if (s != DD_OK) return s;
+  const dd_conf saved = plan->conf[mode];
+  plan->conf[mode] = *conf;
+  auto launch = [&]() -> dd_status {
+    switch (mode) {
+    case DD_SINGLE_STEP:
+      return dd_dedisperse(plan, (const uint8_t *)d_input, (float *)d_output, 0, 0, stream);
+    case DD_STEP_ONE:
+      return dd_subband_step_one(plan, (const uint8_t *)d_input, (float *)d_output, stream);
+    default:
+      return dd_subband_step_two(plan, (const float *)d_input, (float *)d_output, 0, 0, stream);
+    }
+  };
+  std::vector<hipEvent_t> ev(2 * iterations, nullptr);
+  std::vector<float> ms(iterations, 0.0f);
+  hipError_t e = hipSuccess;
+  for (auto &x : ev)
+    if (e == hipSuccess) e = hipEventCreate(&x);
+  for (uint32_t i = 0; i < warmup && e == hipSuccess && s == DD_OK; ++i) s = launch();
+  for (uint32_t i = 0; i < iterations && e == hipSuccess && s == DD_OK; ++i) {
+    e = hipEventRecord(ev[2 * i], (hipStream_t)stream);
+    if (e == hipSuccess) s = launch();
+    if (e == hipSuccess && s == DD_OK) e = hipEventRecord(ev[2 * i + 1], (hipStream_t)stream);
+  }
+  if (e == hipSuccess && s == DD_OK) e = hipStreamSynchronize((hipStream_t)stream);
+  for (uint32_t i = 0; i < iterations && e == hipSuccess && s == DD_OK; ++i)
+    e = hipEventElapsedTime(&ms[i], ev[2 * i], ev[2 * i + 1]);
+  for (auto &x : ev)
+    if (x) (void)hipEventDestroy(x);
+  plan->conf[mode] = saved;
+  if (s != DD_OK) return s;
+  if (e != hipSuccess) return hip_fail(e, "timing launches");
+  double sum = 0, sq = 0;
+  for (float v : ms) sum += v;
+  const double mean = sum / iterations;
+  for (float v : ms) sq += (v - mean) * (v - mean);
+  *mean_ms = (float)mean;
+  *std_ms = (float)std::sqrt(sq / iterations);
   return DD_OK;
 }
 

@@ -1,0 +1,232 @@
+"""Exhaustive tuning harness for the HIP kernels.
+
+Replaces the reference's DedispersionTuning program (src/DedispersionTuning.cpp):
+it enumerates kernel configurations under the same kinds of constraints
+(:201-294, here with HIP limits), times each one with HIP events through
+``dd_tune_time`` (:384-395), reports the reference's metric
+GFLOP/s = giga(adds)/seconds (:327-333, :412) next to DM-trial·samples/s,
+and writes the best configuration per (device, DMs) in the format read by
+readTunedDedispersionConf (src/Dedispersion.cpp:20-81):
+``device nrDMs split local unroll t0 t1 t2 i0 i1 i2``.
+
+Records are JSON lines; a sweep resumes by skipping configurations already
+recorded for the same layout key.  Invalid configurations are recorded with
+``"valid": false`` and the sweep continues (the reference skips a
+configuration that fails to compile or runs out of resources, :343-347,
+:396-409).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import itertools
+import json
+import os
+import sys
+import time
+from dataclasses import asdict
+from typing import Dict, Iterable, List, Optional
+
+from . import DD_SINGLE_STEP, DD_STEP_ONE, DD_STEP_TWO, Conf, DDError, Plan, _lib, load, synth_gaussian_u8
+from . import workloads as wl
+
+MODES = {"single_step": DD_SINGLE_STEP, "step_one": DD_STEP_ONE, "step_two": DD_STEP_TWO}
+
+SPACE = {
+    "threads_d1": (1, 2, 4, 8),
+    "items_d0": (4, 8, 16),
+    "items_d1": (1, 2, 4, 8),
+    "unroll": (1, 2, 4, 8, 16),
+    "local": (0, 1, 2),
+}
+
+
+def enumerate_space(space: Dict = SPACE) -> List[Conf]:
+    confs = []
+    for t1, i0, i1, u, loc in itertools.product(space["threads_d1"], space["items_d0"], space["items_d1"],
+                                                space["unroll"], space["local"]):
+        if t1 * i1 > 64:
+            continue
+        confs.append(Conf(64, t1, i0, i1, u, loc))
+    return confs
+
+
+def layout_of(plan: Plan, mode: int, live_channels: int) -> Dict:
+    """Features of the problem a configuration runs on."""
+    o, z = plan.obs, plan.sizes
+    if mode == DD_SINGLE_STEP:
+        rows, dms, samples, step = live_channels, o.nr_dms, z.nr_samples, o.dm_step
+        beams, groups = o.nr_synthesized_beams, 1
+    elif mode == DD_STEP_ONE:
+        rows, dms, samples, step = live_channels, o.nr_dms_subbanding, z.nr_samples_subbanding, o.dm_step_subbanding
+        beams, groups = o.nr_beams, o.nr_subbands
+    else:
+        rows, dms, samples, step = o.nr_subbands, o.nr_dms, z.nr_samples, o.dm_step
+        beams, groups = o.nr_synthesized_beams * o.nr_dms_subbanding, 1
+    return {
+        "mode": [k for k, v in MODES.items() if v == mode][0],
+        "channels": o.nr_channels,
+        "subbands": o.nr_subbands,
+        "rows": rows,
+        "dms": dms,
+        "samples": samples,
+        "beams": beams,
+        "groups": groups,
+        "dm_step": step,
+        "max_shift": [z.max_shift, z.max_shift_step_one, z.max_shift_step_two][mode],
+    }
+
+
+def adds_of(layout: Dict) -> int:
+    """Additions of one launch: the reference's op count (Tuning:327-333)."""
+    if layout["mode"] == "step_one":
+        return layout["beams"] * layout["dms"] * layout["rows"] * layout["samples"]
+    return layout["beams"] * layout["dms"] * layout["rows"] * layout["samples"] * layout["groups"]
+
+
+class Bench:
+    """Device buffers and plan of one (workload, mode) layout."""
+
+    def __init__(self, workload: wl.Workload, mode: int, torch, padding: int = 128):
+        self.torch = torch
+        self.workload = workload
+        self.mode = mode
+        self.plan = Plan(workload.obs, padding)
+        flags = [0] * self.plan.pad(workload.obs.nr_channels, 4)
+        for c in workload.zapped:
+            flags[c] = 1
+        self.plan.set_zapped_channels(flags)
+        z = self.plan.sizes
+        o = workload.obs
+        s = torch.cuda.current_stream().cuda_stream
+        self.stream = s
+        if mode == DD_SINGLE_STEP:
+            self.d_in = torch.zeros(z.input_elements, dtype=torch.uint8, device="cuda")
+            synth_gaussian_u8(self.d_in.data_ptr(), o.nr_beams * o.nr_channels, z.nr_dispersed_samples, z.in_pitch,
+                              128.0, 16.0, workload.seed, s)
+            self.d_out = torch.empty(z.output_elements, dtype=torch.float32, device="cuda")
+            units = o.nr_synthesized_beams * o.nr_dms * z.nr_samples
+        elif mode == DD_STEP_ONE:
+            self.d_in = torch.zeros(z.sub_input_elements, dtype=torch.uint8, device="cuda")
+            synth_gaussian_u8(self.d_in.data_ptr(), o.nr_beams * o.nr_channels, z.nr_dispersed_samples_subbanding,
+                              z.sub_in_pitch, 128.0, 16.0, workload.seed, s)
+            self.d_out = torch.empty(z.intermediate_elements, dtype=torch.float32, device="cuda")
+            units = o.nr_beams * o.nr_dms_subbanding * o.nr_subbands * z.nr_samples_subbanding
+        else:
+            self.d_in = torch.rand(z.intermediate_elements, dtype=torch.float32, device="cuda") * 6000.0
+            self.d_out = torch.empty(z.sub_output_elements, dtype=torch.float32, device="cuda")
+            units = o.nr_synthesized_beams * o.nr_dms_subbanding * o.nr_dms * z.nr_samples
+        self.units = units
+        self.layout = layout_of(self.plan, mode, o.nr_channels - len(workload.zapped))
+        torch.cuda.synchronize()
+
+    def time(self, conf: Conf, warmup: int = 2, iterations: int = 5) -> Optional[Dict]:
+        lib = load()
+        c = conf.to_c()
+        mean, std = ctypes.c_float(), ctypes.c_float()
+        status = lib.dd_tune_time(self.plan.handle, self.mode, ctypes.byref(c), self.d_in.data_ptr(),
+                                  self.d_out.data_ptr(), warmup, iterations, self.stream, ctypes.byref(mean),
+                                  ctypes.byref(std))
+        if status != 0:
+            return None
+        return {"ms": mean.value, "std_ms": std.value}
+
+
+def key_of(layout: Dict) -> str:
+    return json.dumps(layout, sort_keys=True)
+
+
+def load_records(path: str) -> List[Dict]:
+    if not path or not os.path.exists(path):
+        return []
+    with open(path) as f:
+        return [json.loads(line) for line in f if line.strip()]
+
+
+def sweep(bench: Bench, confs: Iterable[Conf], records_path: Optional[str] = None, warmup: int = 2,
+          iterations: int = 5, device: str = "MI355X", progress=None) -> List[Dict]:
+    """Times every configuration (skipping those already recorded)."""
+    done = {(r["layout_key"], tuple(r["conf"].values())) for r in load_records(records_path)}
+    lkey = key_of(bench.layout)
+    out = []
+    f = open(records_path, "a") if records_path else None
+    try:
+        for conf in confs:
+            ck = tuple(asdict(conf).values())
+            if (lkey, ck) in done:
+                continue
+            t = bench.time(conf, warmup, iterations)
+            rec = {"device": device, "workload": bench.workload.name, "layout": bench.layout, "layout_key": lkey,
+                   "conf": asdict(conf), "valid": t is not None}
+            if t is not None:
+                rec.update(t)
+                rec["units_per_s"] = bench.units / (t["ms"] / 1e3)
+                rec["gflops"] = adds_of(bench.layout) / (t["ms"] / 1e3) / 1e9
+            out.append(rec)
+            if f:
+                f.write(json.dumps(rec) + "\n")
+                f.flush()
+            if progress:
+                progress(rec)
+    finally:
+        if f:
+            f.close()
+    return out
+
+
+def best_of(records: List[Dict]) -> Optional[Dict]:
+    valid = [r for r in records if r.get("valid")]
+    return min(valid, key=lambda r: r["ms"]) if valid else None
+
+
+def tuned_conf_line(device: str, nr_dms: int, conf: Dict) -> str:
+    """One line of the reference's tuned-configuration file."""
+    c = Conf(**conf)
+    return f"{device} {nr_dms} {c.print()}"
+
+
+def write_tuned_file(path: str, entries: List[str]) -> None:
+    with open(path, "w") as f:
+        f.write("# device nrDMs split local unroll threadsD0 threadsD1 threadsD2 itemsD0 itemsD1 itemsD2\n")
+        for e in entries:
+            f.write(e + "\n")
+
+
+WORKLOADS = {
+    "plain-128": (wl.plain_128, ["single_step"]),
+    "arts-single": (wl.arts_single, ["single_step"]),
+    "arts-two-step": (wl.arts_two_step, ["step_one", "step_two"]),
+}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description="exhaustive HIP kernel tuning sweep")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="arts-two-step")
+    ap.add_argument("--modes", default=None, help="comma list of single_step,step_one,step_two")
+    ap.add_argument("--records", default=None, help="JSON-lines file to append to (resumable)")
+    ap.add_argument("--tuned", default=None, help="write the best configurations in the reference format")
+    ap.add_argument("--iterations", type=int, default=5)
+    ap.add_argument("--device", default="MI355X")
+    args = ap.parse_args(argv)
+    import torch
+
+    make, modes = WORKLOADS[args.workload]
+    if args.modes:
+        modes = args.modes.split(",")
+    w = make()
+    lines = []
+    for m in modes:
+        bench = Bench(w, MODES[m], torch)
+        t0 = time.time()
+        recs = sweep(bench, enumerate_space(), args.records, iterations=args.iterations, device=args.device)
+        b = best_of(recs)
+        print(f"{w.name} {m}: {len(recs)} configurations in {time.time() - t0:.1f}s; best "
+              f"{b['conf'] if b else None} {b['ms'] if b else None} ms", file=sys.stderr, flush=True)
+        if b:
+            lines.append(tuned_conf_line(args.device, bench.layout["dms"], b["conf"]))
+    if args.tuned:
+        write_tuned_file(args.tuned, lines)
+
+
+if __name__ == "__main__":
+    main()

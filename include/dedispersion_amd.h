@@ -73,7 +73,10 @@ typedef struct dd_observation {
  *   items_d0    samples per thread (4, 8 or 16)
  *   items_d1    DMs per thread (1, 2, 4 or 8)
  *   unroll      channels staged per barrier (1..16)
- *   local       1 = stage rows in LDS, 0 = read rows through the caches
+ *   local       0 = read rows through the caches; 1 = stage rows in LDS and
+ *               sum from per-lane register windows (used when unroll <=
+ *               2 * threads_d1 and the shift spreads fit, else as 2);
+ *               2 = stage rows in LDS and read every DM's run from LDS
  *   split_batches must be 0 (unimplemented in the reference too). */
 typedef struct dd_conf {
   uint32_t threads_d0;
@@ -139,8 +142,9 @@ dd_status dd_plan_set_conf(dd_plan *plan, dd_mode mode, const dd_conf *conf);
 dd_status dd_plan_get_conf(const dd_plan *plan, dd_mode mode, dd_conf *conf);
 
 /* Zapped channels, the reference's uint32 flags [pad(C, padding/4)]
- * (src/DedispersionTest.cpp:120); non-zero = zapped.  Host pointer.  Packed
- * into a device bitmask. */
+ * (src/DedispersionTest.cpp:120); non-zero = zapped.  Host pointer.
+ * Compacted into device-resident lists of the live channels (per subband for
+ * step one), which the kernels walk. */
 dd_status dd_plan_set_zapped_channels(dd_plan *plan, const uint32_t *zapped);
 /* Beam mappings (host pointers, NULL = keep): single step
  * uint32 [sBeams][pad(C, padding/4)], step two uint32 [sBeams][pad(Nsb, padding/4)]
@@ -173,6 +177,18 @@ dd_status dd_subband_step_one(dd_plan *plan, const uint8_t *d_input, float *d_in
 dd_status dd_subband_step_two(dd_plan *plan, const float *d_intermediate, float *d_output,
                               uint32_t first_synthesized_beam, uint32_t nr_synthesized_beams,
                               void *stream);
+
+/* ---- tuning harness (replaces src/DedispersionTuning.cpp:301-428) ---- */
+
+/* Times one mode under `conf` without changing the plan's configuration:
+ * `warmup` untimed launches, then `iterations` launches each bracketed by
+ * HIP events on `stream`.  Buffers are those of the launch functions
+ * (step two reads d_input as the intermediate).  Returns the mean and the
+ * standard deviation of the per-launch times in milliseconds;
+ * DD_ERR_CONFIGURATION if the configuration cannot run. */
+dd_status dd_tune_time(dd_plan *plan, dd_mode mode, const dd_conf *conf, const void *d_input,
+                       void *d_output, uint32_t warmup, uint32_t iterations, void *stream,
+                       float *mean_ms, float *std_ms);
 
 /* ---- synthetic inputs (seeded, reproducible) ---- */
 

@@ -61,7 +61,7 @@ def test_single_step_plain_128(torch_cuda):
 
 SWEEP_CONFS = [
     Conf(64, t1, i0, i1, u, loc)
-    for (t1, i0, i1, u, loc) in itertools.product((1, 2, 4, 8), (4, 8, 16), (1, 2, 4, 8), (1, 3, 8), (1, 0))
+    for (t1, i0, i1, u, loc) in itertools.product((1, 2, 4, 8), (4, 8, 16), (1, 2, 4, 8), (1, 3, 8), (1, 2, 0))
     if t1 * i1 <= 64
 ]
 
@@ -162,7 +162,7 @@ def two_step_case(torch, w, padding=128, conf1=None, conf2=None, seed=None):
 
 STEP_ONE_CONFS = [
     Conf(64, t1, i0, i1, u, loc)
-    for (t1, i0, i1, u, loc) in itertools.product((1, 4), (4, 8, 16), (1, 2, 8), (1, 5), (1, 0))
+    for (t1, i0, i1, u, loc) in itertools.product((1, 4), (4, 8, 16), (1, 2, 8), (1, 5), (1, 2, 0))
 ]
 
 
@@ -193,7 +193,7 @@ def test_step_one_conf_sweep(torch_cuda, tiny_two_step, conf):
 
 STEP_TWO_CONFS = [
     Conf(64, t1, i0, i1, u, loc)
-    for (t1, i0, i1, u, loc) in itertools.product((1, 2, 4), (4, 8, 16), (1, 4, 8), (1, 3), (1, 0))
+    for (t1, i0, i1, u, loc) in itertools.product((1, 2, 4), (4, 8, 16), (1, 4, 8), (1, 3), (1, 2, 0))
 ]
 
 
@@ -253,7 +253,7 @@ def test_invalid_configurations_are_rejected(torch_cuda):
     w = workloads.plain_128()
     plan = Plan(w.obs, 128)
     for bad in (Conf(threads_d0=32), Conf(threads_d1=0), Conf(threads_d1=9), Conf(items_d0=5), Conf(items_d1=3),
-                Conf(unroll=0), Conf(unroll=17), Conf(split_batches=1), Conf(threads_d1=8, items_d1=16)):
+                Conf(unroll=0), Conf(unroll=17), Conf(split_batches=1), Conf(threads_d1=8, items_d1=16), Conf(local=3)):
         with pytest.raises(DDError):
             plan.set_conf(DD_SINGLE_STEP, bad)
     with pytest.raises(DDError):
@@ -267,3 +267,52 @@ def test_invalid_configurations_are_rejected(torch_cuda):
     buf = torch_cuda.zeros(16, dtype=torch_cuda.float32, device="cuda")
     with pytest.raises(DDError):
         short.dedisperse(buf.data_ptr(), buf.data_ptr())
+
+
+# --------------------------------------------------- buffer placement (regression)
+
+
+def high_address_view(torch, nbytes: int):
+    """A uint8 view of device memory whose address has bit 31 of its low 32
+    bits set (the buffer resources of the kernels carry 48-bit addresses; a
+    sign-extended low half once corrupted their upper bits)."""
+    big = torch.zeros(2**31 + nbytes + 2**20, dtype=torch.uint8, device="cuda")
+    base = big.data_ptr()
+    low = base & 0xFFFFFFFF
+    off = 0 if low >= 0x80000000 else ((0x80000000 - low + 255) & ~255)
+    view = big[off:off + nbytes]
+    assert (view.data_ptr() & 0x80000000) and view.data_ptr() % 256 == 0
+    return big, view
+
+
+def test_buffers_at_high_addresses(torch_cuda):
+    torch = torch_cuda
+    w = workloads.tiny(channels=96, samples=700, dms=12, step=0.9, subbands=8, coarse=9, coarse_step=4.5,
+                       min_freq=1300.0, bandwidth=1.0, zapped=(3, 50))
+    plan = Plan(w.obs, 128)
+    flags = zapped_flags(plan, w.zapped)
+    plan.set_zapped_channels(flags)
+    z = plan.sizes
+    s = stream_of(torch)
+    keep_in, d_in = high_address_view(torch, z.input_elements)
+    src = make_input(torch, plan, w, z.in_pitch, z.nr_dispersed_samples, 5)
+    d_in.copy_(src)
+    keep_out, d_out = high_address_view(torch, 4 * z.output_elements)
+    plan.dedisperse(d_in.data_ptr(), d_out.data_ptr(), stream=s)
+    keep_sub, d_sub = high_address_view(torch, z.sub_input_elements)
+    d_sub.copy_(make_input(torch, plan, w, z.sub_in_pitch, z.nr_dispersed_samples_subbanding, 6))
+    keep_mid, d_mid = high_address_view(torch, 4 * z.intermediate_elements)
+    keep_two, d_two = high_address_view(torch, 4 * z.sub_output_elements)
+    plan.step_one(d_sub.data_ptr(), d_mid.data_ptr(), stream=s)
+    plan.step_two(d_mid.data_ptr(), d_two.data_ptr(), stream=s)
+    torch.cuda.synchronize()
+    orc = oracle.Oracle(w.obs, 128, threads=host_threads())
+    want = orc.dedispersion(d_in.cpu().numpy(), flags)
+    compare_rows(d_out.view(torch.float32).cpu().numpy().reshape(want.shape), want, w.obs.nr_samples_per_batch,
+                 "single step at high addresses")
+    want_mid = orc.step_one(d_sub.cpu().numpy(), flags)
+    mid = d_mid.view(torch.float32).cpu().numpy().reshape(want_mid.shape)
+    compare_rows(mid, want_mid, z.nr_samples_subbanding, "step one at high addresses")
+    want_two = orc.step_two(mid.ravel())
+    compare_rows(d_two.view(torch.float32).cpu().numpy().reshape(want_two.shape), want_two,
+                 w.obs.nr_samples_per_batch, "step two at high addresses")

@@ -40,7 +40,7 @@ struct RowSumU8Params {
   uint32_t win_bytes;         // LDS window per row, bytes (multiple of 16)
   uint32_t nontemporal;       // stream the output past the caches
   uint32_t vec_store;         // output rows 16-byte aligned
-  // register-window kernels: shifts computed from the float table
+  // pipelined kernels: shifts computed from the float table
   const float *table;         // float delay per row (samples per unit DM)
   float first_dm, dm_step;
   uint32_t n_tiles_x, n_tiles_y, n_tiles;  // 1-D grid decomposition
@@ -84,8 +84,7 @@ struct Geometry {
   uint32_t local;      // LDS staging
   uint32_t win;        // LDS window per row (bytes for uint8 rows, floats for float rows)
   uint32_t lds_bytes;  // dynamic LDS per workgroup
-  uint32_t window;     // 1 = register-window kernel
-  uint32_t span;       // register-window capacity (largest per-wavefront shift spread)
+  uint32_t staged;     // 1 = pipelined (double-buffered) kernel
 };
 
 // Launchers (kernels_u8.hip / kernels_f32.hip).  Return hipSuccess or the
@@ -96,23 +95,43 @@ hipError_t launch_rowsum_f32(const RowSumF32Params &p, const Geometry &g, dim3 g
                              hipStream_t stream);
 bool rowsum_u8_supported(uint32_t items_d0, uint32_t items_d1);
 bool rowsum_f32_supported(uint32_t items_d0, uint32_t items_d1);
-// Register-window kernels (kernels_u8_win.hip / kernels_f32_win.hip).
-hipError_t launch_rowsum_u8_win(const RowSumU8Params &p, const Geometry &g, bool flush, hipStream_t stream);
-hipError_t launch_rowsum_f32_win(const RowSumF32Params &p, const Geometry &g, hipStream_t stream);
-// Smallest supported register-window span >= need, or 0.
-uint32_t u8_win_span(uint32_t need);
-uint32_t f32_win_span(uint32_t need);
-// Geometry constants of the register-window kernels.
-constexpr uint32_t kWinSlotsPerWave = 2;   // rows staged per wavefront per group
-constexpr uint32_t kU8WinChunks = 2;       // 16-byte staging loads per lane per row (window <= 2 KiB)
-inline uint32_t u8_win_reads(uint32_t run, uint32_t span) {  // bytes of a lane window
-  const uint32_t emax = run - 1 + span, nwd = emax / 4 + run / 4 + 1;
-  return ((nwd * 4 + run - 1) / run) * run;
-}
-inline uint32_t f32_win_chunks(uint32_t run) { return run / 4 + 1; }  // 16-byte staging loads per lane per row
-inline uint32_t f32_win_reads(uint32_t run, uint32_t span) {  // floats of a lane window
-  return ((3 + span + run + 3) / 4) * 4;
-}
+// Pipelined kernels (kernels_u8_staged.hip / kernels_f32_staged.hip).
+hipError_t launch_rowsum_u8_staged(const RowSumU8Params &p, const Geometry &g, bool flush, hipStream_t stream);
+hipError_t launch_rowsum_f32_staged(const RowSumF32Params &p, const Geometry &g, hipStream_t stream);
+struct TileDesc {
+  uint32_t job, tile;
+};
+
+// Work queues of a persistent launch.  Tile descriptors (job, logical tile)
+// are split into kQueues contiguous ranges, one per XCD; queue q owns
+// tiles[begin[q], begin[q + 1]).  heads[q * kHeadStride] counts the claims
+// made on queue q and is zero when the launch starts.  A workgroup drains
+// the queue of its own XCD first, then the other queues in turn, and leaves
+// when a claim on the last of them fails: every workgroup makes at most one
+// failed claim per queue, so the launch ends once all tiles are claimed.
+constexpr uint32_t kQueues = 8;
+constexpr uint32_t kHeadStride = 64;  // claim counters 256 bytes apart
+constexpr uint32_t kNoTile = 0xFFFFFFFFu;
+
+struct TileQueues {
+  const TileDesc *tiles;   // (job index, logical tile of that job)
+  const uint32_t *begin;   // [kQueues + 1]
+  uint32_t *heads;         // [kQueues * kHeadStride]
+};
+
+// Persistent batch kernels (kernels_u8_staged.hip / kernels_f32_staged.hip):
+// `workgroups` resident workgroups drain the queues; job j of d_jobs is a
+// staged launch description (n_tiles_x/y set) of one beam or plan.
+hipError_t launch_rowsum_u8_batch(const RowSumU8Params *d_jobs, const TileQueues &q, const Geometry &g, bool flush,
+                                  uint32_t workgroups, hipStream_t stream);
+hipError_t launch_rowsum_f32_batch(const RowSumF32Params *d_jobs, const TileQueues &q, const Geometry &g,
+                                   uint32_t workgroups, hipStream_t stream);
+hipError_t rowsum_u8_batch_occupancy(const Geometry &g, bool flush, int *per_cu);
+hipError_t rowsum_f32_batch_occupancy(const Geometry &g, int *per_cu);
+hipError_t launch_clear_words(uint32_t *d_words, uint32_t n, hipStream_t stream);
+constexpr uint32_t kStagedSlotsPerWave = 2;  // rows staged per wavefront per group
+constexpr uint32_t kU8StagedChunks = 2;      // 16-byte staging loads per lane per row (window <= 2 KiB)
+inline uint32_t f32_staged_chunks(uint32_t run) { return run / 4 + 1; }
 
 // Table kernels (tables.hip).
 struct ShiftFormula {

@@ -1,55 +1,47 @@
-// kernels_u8_win.hip -- register-window shifted row sums over uint8 rows.
+// kernels_u8_staged.hip -- pipelined shifted row sums over uint8 rows.
 //
-// Same operation as kernels_u8.hip (single-step dedispersion,
-// include/Dedispersion.hpp:70-109, and step one, :111-153), organised so that
-// VALU work is the only limit:
+// Same operation and arithmetic as kernels_u8.hip (single-step
+// dedispersion, include/Dedispersion.hpp:70-109, and step one, :111-153), with
+// the per-row overheads taken off the critical path:
 //
-//  * A workgroup owns a (DM tile x sample tile) block; wavefront w owns DPW
-//    consecutive DMs, lane l the RUN consecutive samples t0 + RUN*l + [0, RUN).
 //  * Rows are staged in LDS in groups of `unroll`, double buffered: the
-//    global loads of group k+1 are issued before group k is summed and
-//    written to the other buffer after it, so one barrier per group remains
-//    and the load latency hides behind the additions.
-//  * Per row, a lane reads ONE window of bytes covering the runs of all DPW
-//    DMs of its wavefront with RUN-byte LDS reads at a RUN-byte lane stride
-//    (conflict-free), i.e. about 0.75 LDS bytes per addition.  Each DM then
-//    takes its run out of the registers: the dword offset is wave-uniform
-//    and selected by a scalar branch tree, the byte offset by v_perm_b32
-//    selectors held in SGPRs.
-//  * Integer shifts are computed on the fly from the float delay table with
-//    the reference's unfused arithmetic, once per staged row, and published
-//    to the wavefronts through LDS.
-//  * Workgroups are ordered XCD-aware (window_common.hpp).
+//    global loads of group k+1 are issued before group k is summed and written
+//    to the other buffer after it, so one barrier per group remains and the
+//    load latency hides behind the additions.
+//  * A row's integer shifts for the tile's DMs are computed once, by the lanes
+//    of the wavefront that stages the row, from the float delay table with the
+//    reference's unfused arithmetic, and published through LDS; the summing
+//    wavefronts read them from LDS instead of chasing dependent scalar loads.
+//  * The live rows (and their input beams) of the workgroup are copied to LDS
+//    once at the start.
+//  * Workgroups are ordered XCD-aware (staged_common.hpp).
 //
-// Sums use 16-bit-pair accumulators exactly as kernels_u8.hip; all partial
-// sums are integers below 2^24, so the float output is bit-exact.
-#include "window_common.hpp"
+// The summation is that of kernels_u8.hip: lane l owns the four-sample groups
+// t0 + 4l + 256k, reads the two dwords of a group with one ds_read2_b32 and
+// adds v_perm_b32-extracted 16-bit pairs; partial sums are exact integers.
+#include "staged_common.hpp"
 
 namespace ddamd {
 namespace {
 
-using namespace win;
+using namespace staged;
 
 __device__ __forceinline__ uint32_t sel_even_of(uint32_t r) { return r | 0x0c00u | ((r + 2) << 16) | 0x0c000000u; }
 __device__ __forceinline__ uint32_t sel_odd_of(uint32_t r) { return (r + 1) | 0x0c00u | ((r + 3) << 16) | 0x0c000000u; }
 
-template <int RUN, int DPW, int SPAN, bool FLUSH>
-__global__ void __launch_bounds__(512) rowsum_u8_win(const RowSumU8Params p) {
-  constexpr int NG = RUN / 4;                   // four-sample groups per lane run
-  constexpr int TS = 64 * RUN;                  // samples per tile
-  constexpr int EMAX = RUN - 1 + SPAN;          // largest byte offset of a DM run in a lane window
-  constexpr int NWD = EMAX / 4 + NG + 1;        // dwords a lane window must cover
-  constexpr int NR = (NWD * 4 + RUN - 1) / RUN; // RUN-byte LDS reads per lane window
-  constexpr int NRD = NR * RUN / 4;             // dwords read per lane window
-  constexpr int SPW = kWinSlotsPerWave;
-  constexpr int CPL = kU8WinChunks;
-  extern __shared__ uint32_t lds[];
-
+// One (beam x group, DM tile, sample tile) block: logical tile index
+// `logical` of the launch described by p (DM tile fastest).
+template <int RUN, int DPW, bool FLUSH>
+__device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logical, uint32_t *lds) {
+  constexpr int NG = RUN / 4;   // four-sample groups per lane
+  constexpr int TS = 64 * RUN;  // samples per tile
+  constexpr int SPW = kStagedSlotsPerWave;
+  constexpr int CPL = kU8StagedChunks;
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t wave = uniform(threadIdx.x >> 6);
   const uint32_t nwaves = blockDim.x >> 6;
   const uint32_t TD = nwaves * DPW;
-  const Tile tile = tile_of(blockIdx.x, p.n_tiles_x, p.n_tiles_y, p.n_tiles);
+  const Tile tile = tile_split(logical, p.n_tiles_x, p.n_tiles_y);
   const uint32_t t0 = tile.x * TS;
   const uint32_t d0 = tile.y * TD;
   const uint32_t dw = wave * DPW;  // first DM of this wavefront, relative to d0
@@ -62,7 +54,7 @@ __global__ void __launch_bounds__(512) rowsum_u8_win(const RowSumU8Params p) {
   const uint32_t NCH = WD / 4;          // 16-byte chunks per window
   const uint32_t dmax = p.n_dms - 1;
 
-  // LDS: rows (packed row | beam << 16, float delay) | windows [2][G][WD] | meta [2][G][TD]
+  // LDS: rows (packed row | beam << 16, float delay) | windows [2][G][WD] | shifts [2][G][TD]
   uint32_t *lrow = lds;
   float *ltab = reinterpret_cast<float *>(lds + p.live_pitch);
   uint32_t *windows = lds + ((2 * p.live_pitch + 3) & ~3u);
@@ -91,6 +83,7 @@ __global__ void __launch_bounds__(512) rowsum_u8_win(const RowSumU8Params p) {
         const uint32_t shmin = uniform(sh);
         const uint64_t off = uniform64((uint64_t)in_beam * p.in_beam_pitch + (uint64_t)row * p.in_row_pitch + t0 + shmin);
         const uint64_t a0 = off & ~uint64_t(3);
+        // byte offset of each DM's samples in the window (window starts at a0)
         so[s] = sh - shmin + (uint32_t)(off & 3);
         const uint32_t recs = records_from(p.in_bytes, a0);
         const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(p.in + a0, recs);
@@ -156,50 +149,17 @@ __global__ void __launch_bounds__(512) rowsum_u8_win(const RowSumU8Params p) {
   auto compute = [&](uint32_t buf, uint32_t n) {
     for (uint32_t g = 0; g < n; ++g) {
       const uint32_t *mrow = meta + (buf * G + g) * TD + dw;
-      uint32_t o[DPW];
-#pragma unroll
-      for (int j = 0; j < DPW; ++j) o[j] = uniform(mrow[j]);
-      const uint32_t base = o[0] & ~uint32_t(RUN - 1);
-      const uint32_t *src = windows + (buf * G + g) * WD + (RUN * lane + base) / 4;
-      uint32_t R[NRD];
-#pragma unroll
-      for (int c = 0; c < NR; ++c) {
-        if constexpr (RUN == 16) {
-          const u32x4 v = *reinterpret_cast<const u32x4 *>(src + 4 * c);
-          R[4 * c + 0] = v.x;
-          R[4 * c + 1] = v.y;
-          R[4 * c + 2] = v.z;
-          R[4 * c + 3] = v.w;
-        } else if constexpr (RUN == 8) {
-          const u32x2 v = *reinterpret_cast<const u32x2 *>(src + 2 * c);
-          R[2 * c + 0] = v.x;
-          R[2 * c + 1] = v.y;
-        } else {
-          R[c] = src[c];
-        }
-      }
-      uint32_t q[DPW], se[DPW], sd[DPW];
+      const uint32_t *win = windows + (buf * G + g) * WD + lane;
 #pragma unroll
       for (int j = 0; j < DPW; ++j) {
-        const uint32_t e = o[j] - base;
-        q[j] = e >> 2;
-        se[j] = sel_even_of(e & 3);
-        sd[j] = sel_odd_of(e & 3);
-      }
-      // Every DM takes its run at its own wave-uniform dword offset; walking
-      // the offsets at compile time keeps each update a plain conditional add.
+        const uint32_t o = uniform(mrow[j]);
+        const uint32_t *src = win + (o >> 2);
+        const uint32_t se = sel_even_of(o & 3), sd = sel_odd_of(o & 3);
 #pragma unroll
-      for (int x = 0; x <= EMAX / 4; ++x) {
-#pragma unroll
-        for (int j = 0; j < DPW; ++j) {
-          if (q[j] == (uint32_t)x) {
-            asm volatile("");
-#pragma unroll
-            for (int k = 0; k < NG; ++k) {
-              even[j][k] += __builtin_amdgcn_perm(R[x + k + 1], R[x + k], se[j]);
-              odd[j][k] += __builtin_amdgcn_perm(R[x + k + 1], R[x + k], sd[j]);
-            }
-          }
+        for (int k = 0; k < NG; ++k) {
+          const uint32_t lo = src[64 * k], hi = src[64 * k + 1];
+          even[j][k] += __builtin_amdgcn_perm(hi, lo, se);
+          odd[j][k] += __builtin_amdgcn_perm(hi, lo, sd);
         }
       }
     }
@@ -236,7 +196,7 @@ __global__ void __launch_bounds__(512) rowsum_u8_win(const RowSumU8Params p) {
 #pragma unroll
   for (int j = 0; j < DPW; ++j) {
     const uint32_t dm = d0 + dw + j;
-    if (dm >= p.n_dms) break;
+    if (dm >= p.n_dms) continue;
     float *orow = p.out + (uint64_t)beam * p.out_beam_pitch + (uint64_t)group * p.out_group_pitch +
                   (uint64_t)dm * p.out_dm_pitch;
 #pragma unroll
@@ -251,7 +211,7 @@ __global__ void __launch_bounds__(512) rowsum_u8_win(const RowSumU8Params p) {
         v[2] = (float)(even[j][k] >> 16);
         v[3] = (float)(odd[j][k] >> 16);
       }
-      const uint32_t t = t0 + RUN * lane + 4 * k;
+      const uint32_t t = t0 + 4 * lane + 256 * k;
       if (p.vec_store && t + 3 < p.n_samples) {
         f32x4 f = {v[0], v[1], v[2], v[3]};
         if (p.nontemporal) __builtin_nontemporal_store(f, reinterpret_cast<f32x4 *>(orow + t));
@@ -265,54 +225,74 @@ __global__ void __launch_bounds__(512) rowsum_u8_win(const RowSumU8Params p) {
   }
 }
 
-template <int RUN, int DPW, int SPAN, bool FLUSH>
-hipError_t launch_one(const RowSumU8Params &p, const Geometry &g, hipStream_t s) {
-  hipLaunchKernelGGL((rowsum_u8_win<RUN, DPW, SPAN, FLUSH>), dim3(p.n_tiles), dim3(64 * g.threads_d1), g.lds_bytes,
-                     s, p);
-  return hipGetLastError();
-}
-
 template <int RUN, int DPW, bool FLUSH>
-hipError_t by_span(const RowSumU8Params &p, const Geometry &g, hipStream_t s) {
-  switch (g.span) {
-  case 8: return launch_one<RUN, DPW, 8, FLUSH>(p, g, s);
-  case 16: return launch_one<RUN, DPW, 16, FLUSH>(p, g, s);
-  case 32: return launch_one<RUN, DPW, 32, FLUSH>(p, g, s);
-  default: return hipErrorInvalidValue;
+__global__ void __launch_bounds__(512) rowsum_u8_staged(const RowSumU8Params p) {
+  extern __shared__ uint32_t lds[];
+  u8_tile<RUN, DPW, FLUSH>(p, xcd_logical(blockIdx.x, p.n_tiles), lds);
+}
+
+// Persistent form: workgroups take (beam, tile) descriptors from XCD-local
+// work queues until every queue is empty (see staged_common.hpp).
+template <int RUN, int DPW, bool FLUSH>
+__global__ void __launch_bounds__(512) rowsum_u8_batch(const RowSumU8Params *__restrict__ beams,
+                                                         const TileQueues q) {
+  extern __shared__ uint32_t lds[];
+  __shared__ uint32_t claimed;
+  QueueCursor cur;
+  for (;;) {
+    const uint32_t t = cur.next(q, &claimed);
+    if (t == kNoTile) break;
+    const TileDesc d = q.tiles[t];
+    u8_tile<RUN, DPW, FLUSH>(beams[d.job], d.tile, lds);
   }
 }
 
-template <int RUN, bool FLUSH>
-hipError_t by_dpw(const RowSumU8Params &p, const Geometry &g, hipStream_t s) {
-  switch (g.items_d1) {
-  case 1: return by_span<RUN, 1, FLUSH>(p, g, s);
-  case 2: return by_span<RUN, 2, FLUSH>(p, g, s);
-  case 4: return by_span<RUN, 4, FLUSH>(p, g, s);
-  case 8: return by_span<RUN, 8, FLUSH>(p, g, s);
-  default: return hipErrorInvalidValue;
-  }
-}
-
-template <bool FLUSH>
-hipError_t by_run(const RowSumU8Params &p, const Geometry &g, hipStream_t s) {
-  switch (g.items_d0) {
-  case 4: return by_dpw<4, FLUSH>(p, g, s);
-  case 8: return by_dpw<8, FLUSH>(p, g, s);
-  case 16: return by_dpw<16, FLUSH>(p, g, s);
-  default: return hipErrorInvalidValue;
-  }
+// Calls f(RUN, DPW, FLUSH) with the compile-time constants of geometry g.
+template <typename F> hipError_t dispatch(const Geometry &g, bool flush, F &&f) {
+  auto by_dpw = [&](auto run, auto fl) -> hipError_t {
+    switch (g.items_d1) {
+    case 1: return f(run, std::integral_constant<int, 1>{}, fl);
+    case 2: return f(run, std::integral_constant<int, 2>{}, fl);
+    case 4: return f(run, std::integral_constant<int, 4>{}, fl);
+    case 8: return f(run, std::integral_constant<int, 8>{}, fl);
+    default: return hipErrorInvalidValue;
+    }
+  };
+  auto by_run = [&](auto fl) -> hipError_t {
+    switch (g.items_d0) {
+    case 4: return by_dpw(std::integral_constant<int, 4>{}, fl);
+    case 8: return by_dpw(std::integral_constant<int, 8>{}, fl);
+    case 16: return by_dpw(std::integral_constant<int, 16>{}, fl);
+    default: return hipErrorInvalidValue;
+    }
+  };
+  return flush ? by_run(std::true_type{}) : by_run(std::false_type{});
 }
 
 } // namespace
 
-uint32_t u8_win_span(uint32_t need) {
-  for (uint32_t s : {8u, 16u, 32u})
-    if (need <= s) return s;
-  return 0;
+hipError_t launch_rowsum_u8_staged(const RowSumU8Params &p, const Geometry &g, bool flush, hipStream_t stream) {
+  return dispatch(g, flush, [&](auto run, auto dpw, auto fl) {
+    hipLaunchKernelGGL((rowsum_u8_staged<run.value, dpw.value, fl.value>), dim3(p.n_tiles), dim3(64 * g.threads_d1),
+                       g.lds_bytes, stream, p);
+    return hipGetLastError();
+  });
 }
 
-hipError_t launch_rowsum_u8_win(const RowSumU8Params &p, const Geometry &g, bool flush, hipStream_t stream) {
-  return flush ? by_run<true>(p, g, stream) : by_run<false>(p, g, stream);
+hipError_t launch_rowsum_u8_batch(const RowSumU8Params *d_jobs, const TileQueues &q, const Geometry &g, bool flush,
+                                  uint32_t workgroups, hipStream_t stream) {
+  return dispatch(g, flush, [&](auto run, auto dpw, auto fl) {
+    hipLaunchKernelGGL((rowsum_u8_batch<run.value, dpw.value, fl.value>), dim3(workgroups), dim3(64 * g.threads_d1),
+                       g.lds_bytes, stream, d_jobs, q);
+    return hipGetLastError();
+  });
+}
+
+hipError_t rowsum_u8_batch_occupancy(const Geometry &g, bool flush, int *per_cu) {
+  return dispatch(g, flush, [&](auto run, auto dpw, auto fl) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rowsum_u8_batch<run.value, dpw.value, fl.value>,
+                                                        (int)(64 * g.threads_d1), (size_t)g.lds_bytes);
+  });
 }
 
 } // namespace ddamd

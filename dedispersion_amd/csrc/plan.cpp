@@ -12,91 +12,13 @@
 #include <string>
 #include <vector>
 
-#include "internal.hpp"
+#include "plan_internal.hpp"
 
 namespace ddamd {
 
 static thread_local std::string g_last_error;
 
 void set_error(const std::string &message) { g_last_error = message; }
-
-static inline uint32_t padto(uint32_t v, uint32_t k) { return k == 0 ? v : ((v + k - 1) / k) * k; }
-static inline uint32_t roundup(uint32_t v, uint32_t k) { return ((v + k - 1) / k) * k; }
-
-template <typename T> struct DeviceBuffer {
-  T *ptr = nullptr;
-  size_t count = 0;
-  ~DeviceBuffer() { release(); }
-  void release() {
-    if (ptr) (void)hipFree(ptr);
-    ptr = nullptr;
-    count = 0;
-  }
-  hipError_t allocate(size_t n) {
-    release();
-    count = n;
-    if (n == 0) return hipSuccess;
-    return hipMalloc(reinterpret_cast<void **>(&ptr), n * sizeof(T));
-  }
-  hipError_t upload(const std::vector<T> &host) {
-    hipError_t e = allocate(host.size());
-    if (e != hipSuccess || host.empty()) return e;
-    return hipMemcpy(ptr, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice);
-  }
-  hipError_t download(std::vector<T> &host) const {
-    host.resize(count);
-    if (count == 0) return hipSuccess;
-    return hipMemcpy(host.data(), ptr, count * sizeof(T), hipMemcpyDeviceToHost);
-  }
-};
-
-// Integer shift table of one mode, device copy and host copy.
-struct ShiftTable {
-  uint32_t rows = 0, dms = 0, pitch = 0;
-  DeviceBuffer<uint32_t> dev;
-  std::vector<uint32_t> host;
-  uint32_t at(uint32_t r, uint32_t d) const { return host[(size_t)r * pitch + d]; }
-  uint32_t max_shift() const {
-    uint32_t m = 0;
-    for (uint32_t r = 0; r < rows; ++r)
-      for (uint32_t d = 0; d < dms; ++d) m = std::max(m, at(r, d));
-    return m;
-  }
-  // Largest shift spread inside any DM tile of `tile` consecutive DMs.
-  uint32_t max_spread(uint32_t tile) const {
-    uint32_t m = 0;
-    for (uint32_t r = 0; r < rows; ++r)
-      for (uint32_t d0 = 0; d0 < dms; d0 += tile) {
-        uint32_t a = at(r, d0), b = at(r, std::min(d0 + tile - 1, pitch - 1));
-        m = std::max(m, b - a);
-      }
-    return m;
-  }
-};
-
-} // namespace ddamd
-
-using namespace ddamd;
-
-struct dd_plan {
-  dd_observation obs{};
-  uint32_t padding = 0;
-  int device = 0;
-  size_t max_lds = 65536;
-  bool two_step = false;
-  dd_sizes sizes{};
-  dd_conf conf[3]{};
-
-  DeviceBuffer<float> shifts, shifts2, relative;
-  std::vector<float> h_shifts, h_shifts2;
-  ShiftTable ish_single, ish_one, ish_two;
-
-  std::vector<uint32_t> h_zapped;  // flags [pad(C)]
-  DeviceBuffer<uint32_t> live_single, live_single_count, live_sub, live_sub_count;
-  DeviceBuffer<uint32_t> map_single, map_two;
-};
-
-namespace {
 
 dd_status fail(dd_status s, const std::string &msg) {
   set_error(msg);
@@ -107,11 +29,7 @@ dd_status hip_fail(hipError_t e, const char *what) {
   return fail(DD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define DD_HIP(call)                                                                                 \
-  do {                                                                                               \
-    hipError_t e_ = (call);                                                                          \
-    if (e_ != hipSuccess) return hip_fail(e_, #call);                                                \
-  } while (0)
+namespace {
 
 uint32_t cps(const dd_plan *p) { return p->obs.nr_channels / p->obs.nr_subbands; }
 
@@ -208,7 +126,8 @@ dd_conf default_conf(dd_mode mode) {
   return c;
 }
 
-// Geometry of a mode under its configuration; validates limits.
+} // namespace
+
 dd_status geometry_of(const dd_plan *p, dd_mode mode, const dd_conf &c, Geometry &g) {
   if (c.split_batches) return fail(DD_ERR_UNSUPPORTED, "split batches mode is not implemented");
   if (c.threads_d0 != 64) return fail(DD_ERR_CONFIGURATION, "threads_d0 must be 64 (one wavefront along samples)");
@@ -223,52 +142,33 @@ dd_status geometry_of(const dd_plan *p, dd_mode mode, const dd_conf &c, Geometry
   g.local = c.local ? 1 : 0;
   g.win = 0;
   g.lds_bytes = 0;
-  g.window = 0;
-  g.span = 0;
+  g.staged = 0;
   const uint32_t ts = 64 * c.items_d0;
+  const bool stageable = c.local == 1 && c.unroll <= kStagedSlotsPerWave * c.threads_d1;
   if (mode == DD_STEP_TWO) {
     if (!rowsum_f32_supported(c.items_d0, c.items_d1))
       return fail(DD_ERR_CONFIGURATION, "items_d0 must be 4, 8 or 16 and items_d1 1, 2, 4 or 8");
     if (g.local) {
-      const uint32_t tile_spread = p->ish_two.max_spread(tile);
-      if (c.local == 1 && (c.items_d0 == 4 || c.items_d0 == 16) && c.unroll <= kWinSlotsPerWave * c.threads_d1) {
-        const uint32_t span = f32_win_span(p->ish_two.max_spread(c.items_d1));
-        const uint32_t wf = roundup(ts + tile_spread + f32_win_reads(c.items_d0, span), 16);
-        if (span && wf / 4 <= 64 * f32_win_chunks(c.items_d0)) {
-          g.window = 1;
-          g.span = span;
-          g.win = wf;
-          const uint32_t wp = c.items_d0 == 16 ? wf + wf / 4 : wf;
-          g.lds_bytes = 4 * (roundup(2 * p->obs.nr_subbands, 4) + 2 * c.unroll * wp + 2 * c.unroll * tile);
-        }
-      }
-      if (!g.window) {
-        // lane reads floats [spread + lane + 64k] for k < items_d0
-        g.win = roundup(ts + tile_spread, 4);
-        g.lds_bytes = c.unroll * g.win * 4;
+      // a lane reads floats [spread + lane + 64k] for k < items_d0
+      g.win = roundup(ts + p->ish_two.max_spread(tile), 4);
+      g.lds_bytes = c.unroll * g.win * 4;
+      if (stageable && g.win / 4 <= 64 * f32_staged_chunks(c.items_d0)) {
+        g.staged = 1;
+        g.lds_bytes = 4 * (roundup(2 * p->obs.nr_subbands, 4) + 2 * c.unroll * g.win + 2 * c.unroll * tile);
       }
     }
   } else {
     if (!rowsum_u8_supported(c.items_d0, c.items_d1))
       return fail(DD_ERR_CONFIGURATION, "items_d0 must be 4, 8 or 16 and items_d1 1, 2, 4 or 8");
     if (g.local) {
+      // window start misalignment (< 4) + spread + the dword after each group
       const ShiftTable &t = mode == DD_SINGLE_STEP ? p->ish_single : p->ish_one;
-      const uint32_t tile_spread = t.max_spread(tile);
-      if (c.local == 1 && c.unroll <= kWinSlotsPerWave * c.threads_d1) {
-        const uint32_t span = u8_win_span(t.max_spread(c.items_d1));
-        const uint32_t win = roundup(ts - c.items_d0 + tile_spread + 3 + u8_win_reads(c.items_d0, span), 16);
-        if (span && win <= 16 * 64 * kU8WinChunks) {
-          const uint32_t live_pitch = mode == DD_SINGLE_STEP ? p->obs.nr_channels : cps(p);
-          g.window = 1;
-          g.span = span;
-          g.win = win;
-          g.lds_bytes = 4 * (roundup(2 * live_pitch, 4) + 2 * c.unroll * (win / 4) + 2 * c.unroll * tile);
-        }
-      }
-      if (!g.window) {
-        // window start misalignment (< 4) + spread + the dword after each group
-        g.win = roundup(ts + tile_spread + 8, 16);
-        g.lds_bytes = c.unroll * g.win;
+      g.win = roundup(ts + t.max_spread(tile) + 8, 16);
+      g.lds_bytes = c.unroll * g.win;
+      if (stageable && g.win <= 16 * 64 * kU8StagedChunks) {
+        const uint32_t live_pitch = mode == DD_SINGLE_STEP ? p->obs.nr_channels : cps(p);
+        g.staged = 1;
+        g.lds_bytes = 4 * (roundup(2 * live_pitch, 4) + 2 * c.unroll * (g.win / 4) + 2 * c.unroll * tile);
       }
     }
   }
@@ -277,9 +177,177 @@ dd_status geometry_of(const dd_plan *p, dd_mode mode, const dd_conf &c, Geometry
   return DD_OK;
 }
 
+namespace {
 bool aligned16(const void *ptr) { return (reinterpret_cast<uintptr_t>(ptr) & 15) == 0; }
-
 } // namespace
+
+dd_status prepare_single(dd_plan *plan, const dd_conf &c, const uint8_t *d_input, float *d_output,
+                         uint32_t first_sbeam, uint32_t nr_sbeams, U8Launch &L) {
+  if (!plan || !d_input || !d_output) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  const dd_observation &o = plan->obs;
+  const dd_sizes &z = plan->sizes;
+  if (nr_sbeams == 0) nr_sbeams = o.nr_synthesized_beams - std::min(first_sbeam, o.nr_synthesized_beams);
+  if (nr_sbeams == 0 || first_sbeam + nr_sbeams > o.nr_synthesized_beams)
+    return fail(DD_ERR_INVALID_ARGUMENT, "synthesized beam range out of bounds");
+  if ((uint64_t)z.nr_samples + z.max_shift > z.in_pitch)
+    return fail(DD_ERR_INVALID_ARGUMENT, "dispersed batch shorter than samples + largest shift");
+  dd_status s = geometry_of(plan, DD_SINGLE_STEP, c, L.g);
+  if (s != DD_OK) return s;
+  RowSumU8Params &k = L.k;
+  k = RowSumU8Params{};
+  k.in = d_input;
+  k.in_bytes = z.input_elements;
+  k.out = d_output;
+  k.ishift = plan->ish_single.dev.ptr;
+  k.dm_pitch = plan->ish_single.pitch;
+  k.live = plan->live_single.ptr;
+  k.live_count = plan->live_single_count.ptr;
+  k.live_pitch = o.nr_channels;
+  k.map = plan->map_single.ptr;
+  k.map_pitch = padto(o.nr_channels, plan->padding / 4);
+  k.first_sbeam = first_sbeam;
+  k.in_row_pitch = z.in_pitch;
+  k.in_beam_pitch = (uint64_t)o.nr_channels * z.in_pitch;
+  k.out_dm_pitch = z.out_pitch;
+  k.out_beam_pitch = (uint64_t)o.nr_dms * z.out_pitch;
+  k.out_group_pitch = 0;
+  k.n_samples = z.nr_samples;
+  k.n_dms = o.nr_dms;
+  k.n_groups = 1;
+  k.unroll = c.unroll;
+  k.win_bytes = L.g.win;
+  k.nontemporal = 1;
+  k.vec_store = (z.out_pitch % 4 == 0) && aligned16(d_output);
+  k.table = plan->shifts.ptr;
+  k.first_dm = o.first_dm;
+  k.dm_step = o.dm_step;
+  const uint32_t ts = 64 * c.items_d0, td = c.threads_d1 * c.items_d1;
+  L.grid = dim3((z.nr_samples + ts - 1) / ts, (o.nr_dms + td - 1) / td, nr_sbeams);
+  L.flush = o.nr_channels > 257;
+  k.n_tiles_x = L.grid.x;
+  k.n_tiles_y = L.grid.y;
+  k.n_tiles = L.grid.x * L.grid.y * L.grid.z;
+  return DD_OK;
+}
+
+dd_status prepare_step_one(dd_plan *plan, const dd_conf &c, const uint8_t *d_input, float *d_intermediate,
+                           U8Launch &L) {
+  if (!plan || !d_input || !d_intermediate) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  if (!plan->two_step) return fail(DD_ERR_INVALID_ARGUMENT, "plan has no subband path");
+  const dd_observation &o = plan->obs;
+  const dd_sizes &z = plan->sizes;
+  if ((uint64_t)z.nr_samples_subbanding + z.max_shift_step_one > z.sub_in_pitch)
+    return fail(DD_ERR_INVALID_ARGUMENT, "dispersed batch shorter than samples + largest shift");
+  dd_status s = geometry_of(plan, DD_STEP_ONE, c, L.g);
+  if (s != DD_OK) return s;
+  RowSumU8Params &k = L.k;
+  k = RowSumU8Params{};
+  k.in = d_input;
+  k.in_bytes = z.sub_input_elements;
+  k.out = d_intermediate;
+  k.ishift = plan->ish_one.dev.ptr;
+  k.dm_pitch = plan->ish_one.pitch;
+  k.live = plan->live_sub.ptr;
+  k.live_count = plan->live_sub_count.ptr;
+  k.live_pitch = cps(plan);
+  k.map = nullptr;
+  k.map_pitch = 0;
+  k.first_sbeam = 0;
+  k.in_row_pitch = z.sub_in_pitch;
+  k.in_beam_pitch = (uint64_t)o.nr_channels * z.sub_in_pitch;
+  k.out_dm_pitch = (uint64_t)o.nr_subbands * z.sub_out_pitch;
+  k.out_beam_pitch = (uint64_t)o.nr_dms_subbanding * o.nr_subbands * z.sub_out_pitch;
+  k.out_group_pitch = z.sub_out_pitch;
+  k.n_samples = z.nr_samples_subbanding;
+  k.n_dms = o.nr_dms_subbanding;
+  k.n_groups = o.nr_subbands;
+  k.unroll = c.unroll;
+  k.win_bytes = L.g.win;
+  k.nontemporal = 0;  // step two re-reads the intermediate
+  k.vec_store = (z.sub_out_pitch % 4 == 0) && aligned16(d_intermediate);
+  k.table = plan->relative.ptr;
+  k.first_dm = o.first_dm_subbanding;
+  k.dm_step = o.dm_step_subbanding;
+  const uint32_t ts = 64 * c.items_d0, td = c.threads_d1 * c.items_d1;
+  L.grid = dim3((z.nr_samples_subbanding + ts - 1) / ts, (o.nr_dms_subbanding + td - 1) / td,
+                o.nr_beams * o.nr_subbands);
+  L.flush = cps(plan) > 257;
+  k.n_tiles_x = L.grid.x;
+  k.n_tiles_y = L.grid.y;
+  k.n_tiles = L.grid.x * L.grid.y * L.grid.z;
+  return DD_OK;
+}
+
+dd_status prepare_step_two(dd_plan *plan, const dd_conf &c, const float *d_intermediate, float *d_output,
+                           uint32_t first_sbeam, uint32_t nr_sbeams, F32Launch &L) {
+  if (!plan || !d_intermediate || !d_output) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  if (!plan->two_step) return fail(DD_ERR_INVALID_ARGUMENT, "plan has no subband path");
+  const dd_observation &o = plan->obs;
+  const dd_sizes &z = plan->sizes;
+  if (nr_sbeams == 0) nr_sbeams = o.nr_synthesized_beams - std::min(first_sbeam, o.nr_synthesized_beams);
+  if (nr_sbeams == 0 || first_sbeam + nr_sbeams > o.nr_synthesized_beams)
+    return fail(DD_ERR_INVALID_ARGUMENT, "synthesized beam range out of bounds");
+  if ((uint64_t)z.nr_samples + z.max_shift_step_two > z.sub_out_pitch)
+    return fail(DD_ERR_INVALID_ARGUMENT, "subbanded batch shorter than samples + largest shift");
+  dd_status s = geometry_of(plan, DD_STEP_TWO, c, L.g);
+  if (s != DD_OK) return s;
+  RowSumF32Params &k = L.k;
+  k = RowSumF32Params{};
+  k.in = d_intermediate;
+  k.in_floats = z.intermediate_elements;
+  k.out = d_output;
+  k.ishift = plan->ish_two.dev.ptr;
+  k.dm_pitch = plan->ish_two.pitch;
+  k.map = plan->map_two.ptr;
+  k.map_pitch = padto(o.nr_subbands, plan->padding / 4);
+  k.first_sbeam = first_sbeam;
+  k.n_rows = o.nr_subbands;
+  k.n_coarse = o.nr_dms_subbanding;
+  k.in_row_pitch = z.sub_out_pitch;
+  k.in_coarse_pitch = (uint64_t)o.nr_subbands * z.sub_out_pitch;
+  k.in_beam_pitch = (uint64_t)o.nr_dms_subbanding * o.nr_subbands * z.sub_out_pitch;
+  k.out_dm_pitch = z.out_pitch;
+  k.out_beam_pitch = (uint64_t)o.nr_dms_subbanding * o.nr_dms * z.out_pitch;
+  k.n_samples = z.nr_samples;
+  k.n_dms = o.nr_dms;
+  k.unroll = c.unroll;
+  k.win_floats = L.g.win;
+  k.nontemporal = 1;
+  k.vec_store = (z.out_pitch % 4 == 0) && aligned16(d_output);
+  k.table = plan->shifts2.ptr;
+  k.first_dm = o.first_dm;
+  k.dm_step = o.dm_step;
+  const uint32_t ts = 64 * c.items_d0, td = c.threads_d1 * c.items_d1;
+  L.grid = dim3((z.nr_samples + ts - 1) / ts, (o.nr_dms + td - 1) / td, nr_sbeams * o.nr_dms_subbanding);
+  k.n_tiles_x = L.grid.x;
+  k.n_tiles_y = L.grid.y;
+  k.n_tiles = L.grid.x * L.grid.y * L.grid.z;
+  return DD_OK;
+}
+
+dd_status run(const U8Launch &L, hipStream_t stream) {
+  if (L.g.staged) {
+    DD_HIP(launch_rowsum_u8_staged(L.k, L.g, L.flush, stream));
+    return DD_OK;
+  }
+  if (L.grid.y > 65535 || L.grid.z > 65535) return fail(DD_ERR_UNSUPPORTED, "grid too large");
+  DD_HIP(launch_rowsum_u8(L.k, L.g, L.flush, L.grid, stream));
+  return DD_OK;
+}
+
+dd_status run(const F32Launch &L, hipStream_t stream) {
+  if (L.g.staged) {
+    DD_HIP(launch_rowsum_f32_staged(L.k, L.g, stream));
+    return DD_OK;
+  }
+  if (L.grid.y > 65535 || L.grid.z > 65535) return fail(DD_ERR_UNSUPPORTED, "grid too large");
+  DD_HIP(launch_rowsum_f32(L.k, L.g, L.grid, stream));
+  return DD_OK;
+}
+
+} // namespace ddamd
+
+using namespace ddamd;
 
 extern "C" {
 
@@ -512,166 +580,25 @@ dd_status dd_plan_get_integer_shifts(const dd_plan *plan, dd_mode mode, uint32_t
 
 dd_status dd_dedisperse(dd_plan *plan, const uint8_t *d_input, float *d_output, uint32_t first_sbeam,
                         uint32_t nr_sbeams, void *stream) {
-  if (!plan || !d_input || !d_output) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
-  const dd_observation &o = plan->obs;
-  const dd_sizes &z = plan->sizes;
-  if (nr_sbeams == 0) nr_sbeams = o.nr_synthesized_beams - std::min(first_sbeam, o.nr_synthesized_beams);
-  if (nr_sbeams == 0 || first_sbeam + nr_sbeams > o.nr_synthesized_beams)
-    return fail(DD_ERR_INVALID_ARGUMENT, "synthesized beam range out of bounds");
-  if ((uint64_t)z.nr_samples + z.max_shift > z.in_pitch)
-    return fail(DD_ERR_INVALID_ARGUMENT, "dispersed batch shorter than samples + largest shift");
-  const dd_conf &c = plan->conf[DD_SINGLE_STEP];
-  Geometry g;
-  dd_status s = geometry_of(plan, DD_SINGLE_STEP, c, g);
-  if (s != DD_OK) return s;
-  RowSumU8Params k{};
-  k.in = d_input;
-  k.in_bytes = z.input_elements;
-  k.out = d_output;
-  k.ishift = plan->ish_single.dev.ptr;
-  k.dm_pitch = plan->ish_single.pitch;
-  k.live = plan->live_single.ptr;
-  k.live_count = plan->live_single_count.ptr;
-  k.live_pitch = o.nr_channels;
-  k.map = plan->map_single.ptr;
-  k.map_pitch = padto(o.nr_channels, plan->padding / 4);
-  k.first_sbeam = first_sbeam;
-  k.in_row_pitch = z.in_pitch;
-  k.in_beam_pitch = (uint64_t)o.nr_channels * z.in_pitch;
-  k.out_dm_pitch = z.out_pitch;
-  k.out_beam_pitch = (uint64_t)o.nr_dms * z.out_pitch;
-  k.out_group_pitch = 0;
-  k.n_samples = z.nr_samples;
-  k.n_dms = o.nr_dms;
-  k.n_groups = 1;
-  k.unroll = c.unroll;
-  k.win_bytes = g.win;
-  k.nontemporal = 1;
-  k.vec_store = (z.out_pitch % 4 == 0) && aligned16(d_output);
-  k.table = plan->shifts.ptr;
-  k.first_dm = o.first_dm;
-  k.dm_step = o.dm_step;
-  const uint32_t ts = 64 * c.items_d0, td = c.threads_d1 * c.items_d1;
-  dim3 grid((z.nr_samples + ts - 1) / ts, (o.nr_dms + td - 1) / td, nr_sbeams);
-  const bool flush = o.nr_channels > 257;
-  if (g.window) {
-    k.n_tiles_x = grid.x;
-    k.n_tiles_y = grid.y;
-    k.n_tiles = grid.x * grid.y * grid.z;
-    DD_HIP(launch_rowsum_u8_win(k, g, flush, (hipStream_t)stream));
-    return DD_OK;
-  }
-  if (grid.y > 65535 || grid.z > 65535) return fail(DD_ERR_UNSUPPORTED, "grid too large");
-  DD_HIP(launch_rowsum_u8(k, g, flush, grid, (hipStream_t)stream));
-  return DD_OK;
+  if (!plan) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  U8Launch L;
+  dd_status s = prepare_single(plan, plan->conf[DD_SINGLE_STEP], d_input, d_output, first_sbeam, nr_sbeams, L);
+  return s != DD_OK ? s : run(L, (hipStream_t)stream);
 }
 
 dd_status dd_subband_step_one(dd_plan *plan, const uint8_t *d_input, float *d_intermediate, void *stream) {
-  if (!plan || !d_input || !d_intermediate) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
-  if (!plan->two_step) return fail(DD_ERR_INVALID_ARGUMENT, "plan has no subband path");
-  const dd_observation &o = plan->obs;
-  const dd_sizes &z = plan->sizes;
-  if ((uint64_t)z.nr_samples_subbanding + z.max_shift_step_one > z.sub_in_pitch)
-    return fail(DD_ERR_INVALID_ARGUMENT, "dispersed batch shorter than samples + largest shift");
-  const dd_conf &c = plan->conf[DD_STEP_ONE];
-  Geometry g;
-  dd_status s = geometry_of(plan, DD_STEP_ONE, c, g);
-  if (s != DD_OK) return s;
-  RowSumU8Params k{};
-  k.in = d_input;
-  k.in_bytes = z.sub_input_elements;
-  k.out = d_intermediate;
-  k.ishift = plan->ish_one.dev.ptr;
-  k.dm_pitch = plan->ish_one.pitch;
-  k.live = plan->live_sub.ptr;
-  k.live_count = plan->live_sub_count.ptr;
-  k.live_pitch = cps(plan);
-  k.map = nullptr;
-  k.map_pitch = 0;
-  k.first_sbeam = 0;
-  k.in_row_pitch = z.sub_in_pitch;
-  k.in_beam_pitch = (uint64_t)o.nr_channels * z.sub_in_pitch;
-  k.out_dm_pitch = (uint64_t)o.nr_subbands * z.sub_out_pitch;
-  k.out_beam_pitch = (uint64_t)o.nr_dms_subbanding * o.nr_subbands * z.sub_out_pitch;
-  k.out_group_pitch = z.sub_out_pitch;
-  k.n_samples = z.nr_samples_subbanding;
-  k.n_dms = o.nr_dms_subbanding;
-  k.n_groups = o.nr_subbands;
-  k.unroll = c.unroll;
-  k.win_bytes = g.win;
-  k.nontemporal = 0;  // step two re-reads the intermediate
-  k.vec_store = (z.sub_out_pitch % 4 == 0) && aligned16(d_intermediate);
-  k.table = plan->relative.ptr;
-  k.first_dm = o.first_dm_subbanding;
-  k.dm_step = o.dm_step_subbanding;
-  const uint32_t ts = 64 * c.items_d0, td = c.threads_d1 * c.items_d1;
-  dim3 grid((z.nr_samples_subbanding + ts - 1) / ts, (o.nr_dms_subbanding + td - 1) / td,
-            o.nr_beams * o.nr_subbands);
-  const bool flush = cps(plan) > 257;
-  if (g.window) {
-    k.n_tiles_x = grid.x;
-    k.n_tiles_y = grid.y;
-    k.n_tiles = grid.x * grid.y * grid.z;
-    DD_HIP(launch_rowsum_u8_win(k, g, flush, (hipStream_t)stream));
-    return DD_OK;
-  }
-  if (grid.y > 65535 || grid.z > 65535) return fail(DD_ERR_UNSUPPORTED, "grid too large");
-  DD_HIP(launch_rowsum_u8(k, g, flush, grid, (hipStream_t)stream));
-  return DD_OK;
+  if (!plan) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  U8Launch L;
+  dd_status s = prepare_step_one(plan, plan->conf[DD_STEP_ONE], d_input, d_intermediate, L);
+  return s != DD_OK ? s : run(L, (hipStream_t)stream);
 }
 
 dd_status dd_subband_step_two(dd_plan *plan, const float *d_intermediate, float *d_output, uint32_t first_sbeam,
                               uint32_t nr_sbeams, void *stream) {
-  if (!plan || !d_intermediate || !d_output) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
-  if (!plan->two_step) return fail(DD_ERR_INVALID_ARGUMENT, "plan has no subband path");
-  const dd_observation &o = plan->obs;
-  const dd_sizes &z = plan->sizes;
-  if (nr_sbeams == 0) nr_sbeams = o.nr_synthesized_beams - std::min(first_sbeam, o.nr_synthesized_beams);
-  if (nr_sbeams == 0 || first_sbeam + nr_sbeams > o.nr_synthesized_beams)
-    return fail(DD_ERR_INVALID_ARGUMENT, "synthesized beam range out of bounds");
-  if ((uint64_t)z.nr_samples + z.max_shift_step_two > z.sub_out_pitch)
-    return fail(DD_ERR_INVALID_ARGUMENT, "subbanded batch shorter than samples + largest shift");
-  const dd_conf &c = plan->conf[DD_STEP_TWO];
-  Geometry g;
-  dd_status s = geometry_of(plan, DD_STEP_TWO, c, g);
-  if (s != DD_OK) return s;
-  RowSumF32Params k{};
-  k.in = d_intermediate;
-  k.in_floats = z.intermediate_elements;
-  k.out = d_output;
-  k.ishift = plan->ish_two.dev.ptr;
-  k.dm_pitch = plan->ish_two.pitch;
-  k.map = plan->map_two.ptr;
-  k.map_pitch = padto(o.nr_subbands, plan->padding / 4);
-  k.first_sbeam = first_sbeam;
-  k.n_rows = o.nr_subbands;
-  k.n_coarse = o.nr_dms_subbanding;
-  k.in_row_pitch = z.sub_out_pitch;
-  k.in_coarse_pitch = (uint64_t)o.nr_subbands * z.sub_out_pitch;
-  k.in_beam_pitch = (uint64_t)o.nr_dms_subbanding * o.nr_subbands * z.sub_out_pitch;
-  k.out_dm_pitch = z.out_pitch;
-  k.out_beam_pitch = (uint64_t)o.nr_dms_subbanding * o.nr_dms * z.out_pitch;
-  k.n_samples = z.nr_samples;
-  k.n_dms = o.nr_dms;
-  k.unroll = c.unroll;
-  k.win_floats = g.win;
-  k.nontemporal = 1;
-  k.vec_store = (z.out_pitch % 4 == 0) && aligned16(d_output);
-  k.table = plan->shifts2.ptr;
-  k.first_dm = o.first_dm;
-  k.dm_step = o.dm_step;
-  const uint32_t ts = 64 * c.items_d0, td = c.threads_d1 * c.items_d1;
-  dim3 grid((z.nr_samples + ts - 1) / ts, (o.nr_dms + td - 1) / td, nr_sbeams * o.nr_dms_subbanding);
-  if (g.window) {
-    k.n_tiles_x = grid.x;
-    k.n_tiles_y = grid.y;
-    k.n_tiles = grid.x * grid.y * grid.z;
-    DD_HIP(launch_rowsum_f32_win(k, g, (hipStream_t)stream));
-    return DD_OK;
-  }
-  if (grid.y > 65535 || grid.z > 65535) return fail(DD_ERR_UNSUPPORTED, "grid too large");
-  DD_HIP(launch_rowsum_f32(k, g, grid, (hipStream_t)stream));
-  return DD_OK;
+  if (!plan) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  F32Launch L;
+  dd_status s = prepare_step_two(plan, plan->conf[DD_STEP_TWO], d_intermediate, d_output, first_sbeam, nr_sbeams, L);
+  return s != DD_OK ? s : run(L, (hipStream_t)stream);
 }
 
 dd_status dd_tune_time(dd_plan *plan, dd_mode mode, const dd_conf *conf, const void *d_input, void *d_output,

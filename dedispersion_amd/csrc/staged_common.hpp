@@ -1,10 +1,12 @@
-// window_common.hpp -- device helpers shared by the register-window kernels.
+// staged_common.hpp -- device helpers shared by the pipelined (staged) kernels.
 #pragma once
 
 #include "internal.hpp"
 
+#include <type_traits>
+
 namespace ddamd {
-namespace win {
+namespace staged {
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
@@ -56,18 +58,14 @@ __device__ __forceinline__ uint32_t shift_of(float first, float step, uint32_t d
   return (uint32_t)__fmul_rn(__fadd_rn(first, __fmul_rn((float)dm, step)), table);
 }
 
-// Tile of workgroup `id` in a 1-D grid of n tiles (nx sample tiles, ny DM
-// tiles, z fastest-last).  Workgroup ids are placed on the 8 XCDs round robin
-// (XCD = id % 8); logical tiles are dealt to XCDs in contiguous chunks, DM
-// tile fastest, so the DM tiles of one (beam, sample tile) -- which read the
-// same input rows -- run on one XCD and share its L2.
+// Tiles.  A launch of n tiles has nx sample tiles and ny DM tiles per
+// (beam x group) slice; the logical tile index runs DM tile fastest, then
+// sample tile, then slice.
 struct Tile {
   uint32_t x, y, z;
 };
 
-__device__ __forceinline__ Tile tile_of(uint32_t id, uint32_t nx, uint32_t ny, uint32_t n) {
-  const uint32_t per = n / 8, rem = n % 8, xcd = id % 8, k = id / 8;
-  const uint32_t logical = xcd * per + min(xcd, rem) + k;
+__device__ __forceinline__ Tile tile_split(uint32_t logical, uint32_t nx, uint32_t ny) {
   Tile t;
   t.y = uniform(logical % ny);
   t.x = uniform((logical / ny) % nx);
@@ -75,5 +73,44 @@ __device__ __forceinline__ Tile tile_of(uint32_t id, uint32_t nx, uint32_t ny, u
   return t;
 }
 
-} // namespace win
+// Logical tile of workgroup `id` in a 1-D grid of n tiles.  Workgroup ids are
+// placed on the 8 XCDs round robin (XCD = id % 8); logical tiles are dealt to
+// XCDs in contiguous chunks, so the DM tiles of one (beam, sample tile) --
+// which read the same input rows -- run on one XCD and share its L2.
+__device__ __forceinline__ uint32_t xcd_logical(uint32_t id, uint32_t n) {
+  const uint32_t per = n / 8, rem = n % 8, xcd = id % 8, k = id / 8;
+  return uniform(xcd * per + min(xcd, rem) + k);
+}
+
+// Claims on the work queues of a persistent launch (TileQueues).
+struct QueueCursor {
+  uint32_t probe = 0;  // queues found empty so far (thread 0 only)
+
+  // The next tile of this workgroup, or kNoTile.  Called by every thread of
+  // the workgroup; one barrier.  `slot` is a workgroup-shared word; the
+  // caller's tile work must contain a barrier between two calls.
+  __device__ __forceinline__ uint32_t next(const TileQueues &q, uint32_t *slot) {
+    if (threadIdx.x == 0) {
+      const uint32_t home = blockIdx.x % kQueues;
+      uint32_t t = kNoTile;
+      while (probe < kQueues) {
+        const uint32_t qi = (home + probe) % kQueues;
+        const uint32_t b = q.begin[qi], n = q.begin[qi + 1] - b;
+        if (n != 0) {
+          const uint32_t k = atomicAdd(q.heads + qi * kHeadStride, 1u);
+          if (k < n) {
+            t = b + k;
+            break;
+          }
+        }
+        ++probe;
+      }
+      *slot = t;
+    }
+    __syncthreads();
+    return uniform(*slot);
+  }
+};
+
+} // namespace staged
 } // namespace ddamd

@@ -73,10 +73,10 @@ typedef struct dd_observation {
  *   items_d0    samples per thread (4, 8 or 16)
  *   items_d1    DMs per thread (1, 2, 4 or 8)
  *   unroll      channels staged per barrier (1..16)
- *   local       0 = read rows through the caches; 1 = stage rows in LDS and
- *               sum from per-lane register windows (used when unroll <=
- *               2 * threads_d1 and the shift spreads fit, else as 2);
- *               2 = stage rows in LDS and read every DM's run from LDS
+ *   local       0 = read rows through the caches; 1 = stage rows in LDS with
+ *               a double-buffered pipeline (used when unroll <= 2 *
+ *               threads_d1 and the row window fits, else as 2); 2 = stage
+ *               rows in LDS, one group at a time
  *   split_batches must be 0 (unimplemented in the reference too). */
 typedef struct dd_conf {
   uint32_t threads_d0;

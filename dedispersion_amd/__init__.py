@@ -22,6 +22,7 @@ __all__ = [
     "Observation",
     "Conf",
     "Plan",
+    "Batch",
     "DDError",
     "DD_SINGLE_STEP",
     "DD_STEP_ONE",
@@ -209,6 +210,69 @@ class Plan:
                 self.handle, d_beam_input, pitch, row_len, dm_value, t0, width, amplitude, stream or None
             )
         )
+
+
+class Batch:
+    """A dd_batch: jobs of several plans run by one persistent launch per
+    kernel kind (optionally replayed from a HIP graph).  The plans and the
+    device buffers of the jobs must outlive the batch."""
+
+    def __init__(self):
+        self.lib = load()
+        handle = ctypes.c_void_p()
+        check(self.lib.dd_batch_create(ctypes.byref(handle)))
+        self.handle = handle
+        self._keep = []  # plans and buffers referenced by jobs
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.dd_batch_destroy(self.handle)
+            self.handle = None
+        self._keep = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add_single_step(self, plan: Plan, d_input: int, d_output: int, first_sbeam: int = 0, nr_sbeams: int = 0,
+                        keep=()):
+        check(self.lib.dd_batch_add_single_step(self.handle, plan.handle, d_input, d_output, first_sbeam, nr_sbeams))
+        self._keep.append((plan, keep))
+
+    def add_two_step(self, plan: Plan, d_input: int, d_intermediate: int, d_output: int, first_sbeam: int = 0,
+                     nr_sbeams: int = 0, keep=()):
+        check(self.lib.dd_batch_add_two_step(self.handle, plan.handle, d_input, d_intermediate, d_output,
+                                             first_sbeam, nr_sbeams))
+        self._keep.append((plan, keep))
+
+    def set_conf(self, mode: int, conf: Conf) -> None:
+        c = conf.to_c()
+        check(self.lib.dd_batch_set_conf(self.handle, mode, ctypes.byref(c)))
+
+    def get_conf(self, mode: int) -> Conf:
+        c = _lib.DDConf()
+        check(self.lib.dd_batch_get_conf(self.handle, mode, ctypes.byref(c)))
+        return Conf.from_c(c)
+
+    def run(self, use_graph: bool = False, stream: int = 0) -> None:
+        check(self.lib.dd_batch_run(self.handle, 1 if use_graph else 0, stream or None))
+
+    def run_per_job(self, stream: int = 0) -> None:
+        check(self.lib.dd_batch_run_per_job(self.handle, stream or None))
+
+    def info(self) -> dict:
+        i = _lib.DDBatchInfo()
+        check(self.lib.dd_batch_get_info(self.handle, ctypes.byref(i)))
+        return {"jobs": i.jobs, "tiles": list(i.tiles), "workgroups": list(i.workgroups),
+                "lds_bytes": list(i.lds_bytes), "units": i.units}
 
 
 def synth_gaussian_u8(d_data: int, rows: int, cols: int, pitch: int, mean: float, std: float, seed: int,

@@ -86,6 +86,16 @@ class DDSizes(ctypes.Structure):
     ]
 
 
+class DDBatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("jobs", c_uint32),
+        ("tiles", c_uint32 * 3),
+        ("workgroups", c_uint32 * 3),
+        ("lds_bytes", c_uint32 * 3),
+        ("units", c_uint64),
+    ]
+
+
 # name -> (restype, argtypes); every exported symbol of include/dedispersion_amd.h
 PROTOTYPES = {
     "dd_last_error": (c_char_p, []),
@@ -109,6 +119,15 @@ PROTOTYPES = {
         [c_void_p, c_int, POINTER(DDConf), c_void_p, c_void_p, c_uint32, c_uint32, c_void_p, POINTER(c_float),
          POINTER(c_float)],
     ),
+    "dd_batch_create": (c_int, [POINTER(c_void_p)]),
+    "dd_batch_destroy": (c_int, [c_void_p]),
+    "dd_batch_add_single_step": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_uint32]),
+    "dd_batch_add_two_step": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_uint32]),
+    "dd_batch_set_conf": (c_int, [c_void_p, c_int, POINTER(DDConf)]),
+    "dd_batch_get_conf": (c_int, [c_void_p, c_int, POINTER(DDConf)]),
+    "dd_batch_run": (c_int, [c_void_p, c_int, c_void_p]),
+    "dd_batch_run_per_job": (c_int, [c_void_p, c_void_p]),
+    "dd_batch_get_info": (c_int, [c_void_p, POINTER(DDBatchInfo)]),
     "dd_synth_gaussian_u8": (c_int, [c_void_p, c_uint64, c_uint64, c_uint64, c_float, c_float, c_uint64, c_void_p]),
     "dd_synth_uniform_u8": (c_int, [c_void_p, c_uint64, c_uint64, c_uint64, c_uint64, c_void_p]),
     "dd_synth_inject_pulse": (

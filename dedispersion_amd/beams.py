@@ -1,0 +1,90 @@
+"""Device-resident sets of beams for batched two-step dedispersion.
+
+A ``BeamSet`` holds, for every workload of a list (one beam each, sizes free
+to differ), a plan with the workload's zapped channels, the seeded uint8
+input synthesized on the device (noise plus the workload's pulses), and the
+intermediate and output buffers; ``batch`` runs all of them through one
+``Batch`` (one persistent launch per step), ``run_per_job`` through one
+launch per beam and step.  Device memory comes from torch.
+"""
+from __future__ import annotations
+
+from typing import List, Optional
+
+from . import DD_STEP_ONE, DD_STEP_TWO, Batch, Conf, Plan, synth_gaussian_u8, synth_uniform_u8
+from .workloads import PADDING, Workload
+
+
+class BeamSet:
+    def __init__(self, torch, workloads: List[Workload], padding: int = PADDING, confs: Optional[dict] = None,
+                 pulses: bool = True):
+        self.torch = torch
+        self.workloads = workloads
+        self.padding = padding
+        self.plans: List[Plan] = []
+        self.inputs, self.mids, self.outs = [], [], []
+        s = torch.cuda.current_stream().cuda_stream
+        self.stream = s
+        for w in workloads:
+            o = w.obs
+            if o.nr_dms_subbanding == 0:
+                raise ValueError(f"{w.name}: a beam set runs the two-step path")
+            plan = Plan(o, padding)
+            flags = [0] * plan.pad(o.nr_channels, 4)
+            for c in w.zapped:
+                flags[c] = 1
+            plan.set_zapped_channels(flags)
+            z = plan.sizes
+            d_in = torch.zeros(z.sub_input_elements, dtype=torch.uint8, device="cuda")
+            rows = o.nr_beams * o.nr_channels
+            if w.noise == "uniform":
+                synth_uniform_u8(d_in.data_ptr(), rows, z.nr_dispersed_samples_subbanding, z.sub_in_pitch, w.seed, s)
+            else:
+                synth_gaussian_u8(d_in.data_ptr(), rows, z.nr_dispersed_samples_subbanding, z.sub_in_pitch, 128.0,
+                                  16.0, w.seed, s)
+            if pulses:
+                for p in w.pulses:
+                    plan.inject_pulse(d_in.data_ptr(), z.sub_in_pitch, z.nr_dispersed_samples_subbanding, p.dm_value,
+                                      p.t0, p.width, p.amplitude, s)
+            self.plans.append(plan)
+            self.inputs.append(d_in)
+            self.mids.append(torch.empty(z.intermediate_elements, dtype=torch.float32, device="cuda"))
+            self.outs.append(torch.empty(z.sub_output_elements, dtype=torch.float32, device="cuda"))
+        self.batch = Batch()
+        if confs:
+            for mode, conf in confs.items():
+                self.batch.set_conf(mode, conf)
+        for plan, d_in, d_mid, d_out in zip(self.plans, self.inputs, self.mids, self.outs):
+            self.batch.add_two_step(plan, d_in.data_ptr(), d_mid.data_ptr(), d_out.data_ptr())
+        torch.cuda.synchronize()
+
+    def set_conf(self, mode: int, conf: Conf) -> None:
+        self.batch.set_conf(mode, conf)
+
+    def run(self, use_graph: bool = True) -> None:
+        self.batch.run(use_graph, self.stream)
+
+    def run_per_job(self) -> None:
+        self.batch.run_per_job(self.stream)
+
+    def info(self) -> dict:
+        return self.batch.info()
+
+    def units(self) -> int:
+        """DM trials x samples x beams of the final outputs."""
+        return self.info()["units"]
+
+    def bytes_moved(self) -> int:
+        """Compulsory HBM bytes of one pass: input read once, intermediate
+        written and read once, output written once (padded rows)."""
+        total = 0
+        for plan in self.plans:
+            z = plan.sizes
+            total += z.sub_input_elements + 8 * z.intermediate_elements + 4 * z.sub_output_elements
+        return total
+
+    def close(self) -> None:
+        self.batch.close()
+        for p in self.plans:
+            p.close()
+        self.plans, self.inputs, self.mids, self.outs = [], [], [], []

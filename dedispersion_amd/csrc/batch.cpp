@@ -1,0 +1,338 @@
+// batch.cpp -- many plans in one persistent launch per kernel kind.
+//
+// The reference enqueues one kernel per beam and step
+// (src/DedispersionTest.cpp:399-416).  A batch instead lists the tiles of
+// every job of one kernel kind -- (job, logical tile) descriptors, each job
+// a prepared launch of the single-plan entry points -- and runs them with a
+// persistent kernel whose workgroups claim tiles from XCD-local queues
+// (TileQueues).  Each tile is computed by the same device function, with
+// the same parameters, as in the job's own launch, so results are
+// bit-identical to the per-job launches.
+//
+// Launch order on the stream: clear the claim counters, then single-step
+// jobs, step one of the two-step jobs, step two of the two-step jobs.  The
+// kernel boundary between step one and step two orders every intermediate
+// write before its reads.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "plan_internal.hpp"
+
+using namespace ddamd;
+
+namespace {
+
+enum JobKind { kSingleStep = 0, kTwoStep = 1 };
+
+struct Job {
+  dd_plan *plan;
+  JobKind kind;
+  const uint8_t *in;
+  float *mid;
+  float *out;
+  uint32_t first_sbeam, nr_sbeams;
+};
+
+// The launches of one kernel kind over all its jobs.
+struct KindQueue {
+  std::vector<U8Launch> u8;    // single step or step one, one per job of the kind
+  std::vector<F32Launch> f32;  // step two
+  Geometry g{};
+  bool flush = false;
+  uint32_t n_tiles = 0, workgroups = 0;
+  DeviceBuffer<RowSumU8Params> d_u8;
+  DeviceBuffer<RowSumF32Params> d_f32;
+  DeviceBuffer<TileDesc> d_tiles;
+  DeviceBuffer<uint32_t> d_begin;
+};
+
+} // namespace
+
+struct dd_batch {
+  std::vector<Job> jobs;
+  dd_conf conf[3]{};
+  bool built = false;
+  KindQueue q[3];
+  DeviceBuffer<uint32_t> heads;  // claim counters, kQueues * kHeadStride per kind
+  uint64_t units = 0;
+  int cus = 0;
+  hipStream_t capture = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  void invalidate() {
+    built = false;
+    drop_graph();
+  }
+  ~dd_batch() {
+    drop_graph();
+    if (capture) (void)hipStreamDestroy(capture);
+  }
+};
+
+namespace {
+
+dd_conf batch_default_conf(dd_mode mode) {
+  // Staged pipeline shapes that fill the device for ARTS-sized beams.
+  dd_conf c{};
+  c.threads_d0 = 64;
+  c.split_batches = 0;
+  c.local = 1;
+  if (mode == DD_STEP_TWO) {
+    c.threads_d1 = 4;
+    c.items_d0 = 8;
+    c.items_d1 = 4;
+    c.unroll = 8;
+  } else {
+    c.threads_d1 = 8;
+    c.items_d0 = 16;
+    c.items_d1 = 2;
+    c.unroll = 16;
+  }
+  return c;
+}
+
+// Splits n tiles into kQueues contiguous ranges of near-equal length.
+std::vector<uint32_t> queue_bounds(uint32_t n) {
+  std::vector<uint32_t> b(kQueues + 1);
+  for (uint32_t q = 0; q <= kQueues; ++q) b[q] = (uint32_t)(((uint64_t)n * q) / kQueues);
+  return b;
+}
+
+template <typename L> dd_status check_staged(const L &launch, dd_mode mode, size_t job) {
+  if (!launch.g.staged)
+    return fail(DD_ERR_CONFIGURATION, "batch configuration of mode " + std::to_string((int)mode) +
+                                          " does not select the staged pipeline for job " + std::to_string(job) +
+                                          " (needs local = 1, unroll <= 2 * threads_d1 and a window that fits)");
+  return DD_OK;
+}
+
+template <typename L> void merge_geometry(KindQueue &k, const L &launch, bool first) {
+  if (first) {
+    k.g = launch.g;
+  } else {
+    k.g.lds_bytes = std::max(k.g.lds_bytes, launch.g.lds_bytes);
+    k.g.win = std::max(k.g.win, launch.g.win);
+  }
+}
+
+dd_status build(dd_batch *b) {
+  if (b->built) return DD_OK;
+  for (auto &k : b->q) {
+    k.u8.clear();
+    k.f32.clear();
+    k.n_tiles = 0;
+    k.workgroups = 0;
+    k.flush = false;
+  }
+  b->units = 0;
+  if (b->cus == 0) {
+    int dev = 0;
+    DD_HIP(hipGetDevice(&dev));
+    DD_HIP(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev));
+  }
+  for (size_t j = 0; j < b->jobs.size(); ++j) {
+    const Job &job = b->jobs[j];
+    const dd_observation &o = job.plan->obs;
+    const dd_sizes &z = job.plan->sizes;
+    dd_status s;
+    if (job.kind == kSingleStep) {
+      U8Launch L;
+      s = prepare_single(job.plan, b->conf[DD_SINGLE_STEP], job.in, job.out, job.first_sbeam, job.nr_sbeams, L);
+      if (s == DD_OK) s = check_staged(L, DD_SINGLE_STEP, j);
+      if (s != DD_OK) return s;
+      KindQueue &k = b->q[DD_SINGLE_STEP];
+      merge_geometry(k, L, k.u8.empty());
+      k.flush = k.flush || L.flush;
+      k.u8.push_back(L);
+      b->units += (uint64_t)L.grid.z * o.nr_dms * z.nr_samples;
+    } else {
+      U8Launch L1;
+      s = prepare_step_one(job.plan, b->conf[DD_STEP_ONE], job.in, job.mid, L1);
+      if (s == DD_OK) s = check_staged(L1, DD_STEP_ONE, j);
+      if (s != DD_OK) return s;
+      F32Launch L2;
+      s = prepare_step_two(job.plan, b->conf[DD_STEP_TWO], job.mid, job.out, job.first_sbeam, job.nr_sbeams, L2);
+      if (s == DD_OK) s = check_staged(L2, DD_STEP_TWO, j);
+      if (s != DD_OK) return s;
+      KindQueue &k1 = b->q[DD_STEP_ONE];
+      merge_geometry(k1, L1, k1.u8.empty());
+      k1.flush = k1.flush || L1.flush;
+      k1.u8.push_back(L1);
+      KindQueue &k2 = b->q[DD_STEP_TWO];
+      merge_geometry(k2, L2, k2.f32.empty());
+      k2.f32.push_back(L2);
+      // grid.z of step two = sBeams x coarse DMs; units = sBeams x Dc x Df x S
+      b->units += (uint64_t)L2.grid.z * o.nr_dms * z.nr_samples;
+    }
+  }
+  for (int m = 0; m < 3; ++m) {
+    KindQueue &k = b->q[m];
+    std::vector<TileDesc> tiles;
+    const size_t njobs = m == DD_STEP_TWO ? k.f32.size() : k.u8.size();
+    for (size_t j = 0; j < njobs; ++j) {
+      const uint32_t n = m == DD_STEP_TWO ? k.f32[j].k.n_tiles : k.u8[j].k.n_tiles;
+      for (uint32_t t = 0; t < n; ++t) tiles.push_back(TileDesc{(uint32_t)j, t});
+    }
+    if (tiles.size() > 0xFFFFFFF0u) return fail(DD_ERR_UNSUPPORTED, "too many tiles in one batch");
+    k.n_tiles = (uint32_t)tiles.size();
+    if (k.n_tiles == 0) continue;
+    if (k.g.lds_bytes > b->jobs.front().plan->max_lds)
+      return fail(DD_ERR_CONFIGURATION, "batch LDS demand exceeds the device limit");
+    int per_cu = 0;
+    if (m == DD_STEP_TWO) {
+      std::vector<RowSumF32Params> params;
+      for (auto &L : k.f32) params.push_back(L.k);
+      DD_HIP(k.d_f32.upload(params));
+      DD_HIP(rowsum_f32_batch_occupancy(k.g, &per_cu));
+    } else {
+      std::vector<RowSumU8Params> params;
+      for (auto &L : k.u8) params.push_back(L.k);
+      DD_HIP(k.d_u8.upload(params));
+      DD_HIP(rowsum_u8_batch_occupancy(k.g, k.flush, &per_cu));
+    }
+    if (per_cu < 1) return fail(DD_ERR_CONFIGURATION, "batch kernel cannot be resident on a compute unit");
+    const uint32_t resident = (uint32_t)per_cu * (uint32_t)b->cus;
+    k.workgroups = std::min(resident, roundup(k.n_tiles, kQueues));
+    DD_HIP(k.d_tiles.upload(tiles));
+    DD_HIP(k.d_begin.upload(queue_bounds(k.n_tiles)));
+  }
+  if (b->heads.count == 0) DD_HIP(b->heads.allocate(3 * kQueues * kHeadStride));
+  b->built = true;
+  return DD_OK;
+}
+
+dd_status enqueue(dd_batch *b, hipStream_t s) {
+  DD_HIP(launch_clear_words(b->heads.ptr, (uint32_t)b->heads.count, s));
+  for (int m = 0; m < 3; ++m) {
+    KindQueue &k = b->q[m];
+    if (k.n_tiles == 0) continue;
+    TileQueues tq{k.d_tiles.ptr, k.d_begin.ptr, b->heads.ptr + m * kQueues * kHeadStride};
+    if (m == DD_STEP_TWO) DD_HIP(launch_rowsum_f32_batch(k.d_f32.ptr, tq, k.g, k.workgroups, s));
+    else DD_HIP(launch_rowsum_u8_batch(k.d_u8.ptr, tq, k.g, k.flush, k.workgroups, s));
+  }
+  return DD_OK;
+}
+
+dd_status add_job(dd_batch *b, const Job &job) {
+  b->jobs.push_back(job);
+  b->invalidate();
+  return DD_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+dd_status dd_batch_create(dd_batch **batch) {
+  if (!batch) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  dd_batch *b = new dd_batch();
+  for (int m = 0; m < 3; ++m) b->conf[m] = batch_default_conf((dd_mode)m);
+  *batch = b;
+  return DD_OK;
+}
+
+dd_status dd_batch_destroy(dd_batch *batch) {
+  delete batch;
+  return DD_OK;
+}
+
+dd_status dd_batch_add_single_step(dd_batch *batch, dd_plan *plan, const uint8_t *d_input, float *d_output,
+                                   uint32_t first_sbeam, uint32_t nr_sbeams) {
+  if (!batch || !plan || !d_input || !d_output) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  return add_job(batch, Job{plan, kSingleStep, d_input, nullptr, d_output, first_sbeam, nr_sbeams});
+}
+
+dd_status dd_batch_add_two_step(dd_batch *batch, dd_plan *plan, const uint8_t *d_input, float *d_intermediate,
+                                float *d_output, uint32_t first_sbeam, uint32_t nr_sbeams) {
+  if (!batch || !plan || !d_input || !d_intermediate || !d_output)
+    return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  if (!plan->two_step) return fail(DD_ERR_INVALID_ARGUMENT, "plan has no subband path");
+  return add_job(batch, Job{plan, kTwoStep, d_input, d_intermediate, d_output, first_sbeam, nr_sbeams});
+}
+
+dd_status dd_batch_set_conf(dd_batch *batch, dd_mode mode, const dd_conf *conf) {
+  if (!batch || !conf || (int)mode < 0 || (int)mode > 2) return fail(DD_ERR_INVALID_ARGUMENT, "invalid argument");
+  batch->conf[mode] = *conf;
+  batch->invalidate();
+  return DD_OK;
+}
+
+dd_status dd_batch_get_conf(const dd_batch *batch, dd_mode mode, dd_conf *conf) {
+  if (!batch || !conf || (int)mode < 0 || (int)mode > 2) return fail(DD_ERR_INVALID_ARGUMENT, "invalid argument");
+  *conf = batch->conf[mode];
+  return DD_OK;
+}
+
+dd_status dd_batch_run(dd_batch *batch, int use_graph, void *stream) {
+  if (!batch) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  if (batch->jobs.empty()) return DD_OK;
+  dd_status s = build(batch);
+  if (s != DD_OK) return s;
+  if (!use_graph) return enqueue(batch, (hipStream_t)stream);
+  if (!batch->exec) {
+    if (!batch->capture) DD_HIP(hipStreamCreateWithFlags(&batch->capture, hipStreamNonBlocking));
+    DD_HIP(hipStreamBeginCapture(batch->capture, hipStreamCaptureModeThreadLocal));
+    s = enqueue(batch, batch->capture);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(batch->capture, &g);
+    if (s != DD_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return s;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    batch->graph = g;
+    e = hipGraphInstantiate(&batch->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      batch->drop_graph();
+      return hip_fail(e, "hipGraphInstantiate");
+    }
+  }
+  DD_HIP(hipGraphLaunch(batch->exec, (hipStream_t)stream));
+  return DD_OK;
+}
+
+dd_status dd_batch_run_per_job(dd_batch *batch, void *stream) {
+  if (!batch) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  dd_status s = build(batch);
+  if (s != DD_OK) return s;
+  size_t i_single = 0, i_two = 0;
+  for (const Job &job : batch->jobs) {
+    if (job.kind == kSingleStep) {
+      s = run(batch->q[DD_SINGLE_STEP].u8[i_single++], (hipStream_t)stream);
+    } else {
+      s = run(batch->q[DD_STEP_ONE].u8[i_two], (hipStream_t)stream);
+      if (s == DD_OK) s = run(batch->q[DD_STEP_TWO].f32[i_two], (hipStream_t)stream);
+      ++i_two;
+    }
+    if (s != DD_OK) return s;
+  }
+  return DD_OK;
+}
+
+dd_status dd_batch_get_info(dd_batch *batch, dd_batch_info *info) {
+  if (!batch || !info) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  *info = dd_batch_info{};
+  info->jobs = (uint32_t)batch->jobs.size();
+  if (batch->jobs.empty()) return DD_OK;
+  dd_status s = build(batch);
+  if (s != DD_OK) return s;
+  for (int m = 0; m < 3; ++m) {
+    info->tiles[m] = batch->q[m].n_tiles;
+    info->workgroups[m] = batch->q[m].workgroups;
+    info->lds_bytes[m] = batch->q[m].n_tiles ? batch->q[m].g.lds_bytes : 0;
+  }
+  info->units = batch->units;
+  return DD_OK;
+}
+
+} // extern "C"

@@ -190,6 +190,53 @@ dd_status dd_tune_time(dd_plan *plan, dd_mode mode, const dd_conf *conf, const v
                        void *d_output, uint32_t warmup, uint32_t iterations, void *stream,
                        float *mean_ms, float *std_ms);
 
+/* ---- batches: many plans in one persistent launch per kernel ----
+ *
+ * The reference launches one kernel per beam and step
+ * (src/DedispersionTest.cpp:399-416, one enqueueNDRangeKernel per call).  A
+ * batch gathers jobs -- the single-step or two-step path of a plan over
+ * device buffers, plans free to differ in DMs and samples -- and runs every
+ * tile of every job of one kernel kind in a single persistent launch over
+ * work queues: one launch for all single-step jobs, one for all step-one
+ * jobs, then one for all step-two jobs.  Results are bit-identical to
+ * calling dd_dedisperse / dd_subband_step_one / dd_subband_step_two on each
+ * job with the same configuration.  One configuration per kernel kind
+ * applies to every job (DD_SINGLE_STEP, DD_STEP_ONE, DD_STEP_TWO); it must
+ * select the staged pipeline (local = 1, unroll <= 2 * threads_d1, and a
+ * window that fits) for every job, or dd_batch_run fails with
+ * DD_ERR_CONFIGURATION.  Plans and buffers must stay alive and unchanged
+ * while they are in a batch.  A batch is used by one host thread at a time. */
+typedef struct dd_batch dd_batch;
+
+typedef struct dd_batch_info {
+  uint32_t jobs;
+  uint32_t tiles[3];        /* tiles per kernel kind (dd_mode order) */
+  uint32_t workgroups[3];   /* resident workgroups of each persistent launch */
+  uint32_t lds_bytes[3];
+  uint64_t units;           /* output elements of the final arrays (DM trials x samples x beams) */
+} dd_batch_info;
+
+dd_status dd_batch_create(dd_batch **batch);
+dd_status dd_batch_destroy(dd_batch *batch);
+/* Adds the single-step path of `plan`: synthesized beams
+ * [first_sbeam, first_sbeam + nr_sbeams) (nr_sbeams 0 = to the last). */
+dd_status dd_batch_add_single_step(dd_batch *batch, dd_plan *plan, const uint8_t *d_input, float *d_output,
+                                   uint32_t first_sbeam, uint32_t nr_sbeams);
+/* Adds the two-step path of `plan`: step one into d_intermediate, then step
+ * two of the synthesized beams [first_sbeam, first_sbeam + nr_sbeams). */
+dd_status dd_batch_add_two_step(dd_batch *batch, dd_plan *plan, const uint8_t *d_input, float *d_intermediate,
+                                float *d_output, uint32_t first_sbeam, uint32_t nr_sbeams);
+dd_status dd_batch_set_conf(dd_batch *batch, dd_mode mode, const dd_conf *conf);
+dd_status dd_batch_get_conf(const dd_batch *batch, dd_mode mode, dd_conf *conf);
+/* Builds the work queues (first run, or after a change) and enqueues the
+ * batch on `stream`.  use_graph != 0: the launches are captured once into a
+ * HIP graph that later runs replay; the graph is rebuilt after any change. */
+dd_status dd_batch_run(dd_batch *batch, int use_graph, void *stream);
+/* The same work as one ordinary launch per job and kernel kind, in job
+ * order (the reference's launch pattern), with the batch configurations. */
+dd_status dd_batch_run_per_job(dd_batch *batch, void *stream);
+dd_status dd_batch_get_info(dd_batch *batch, dd_batch_info *info);
+
 /* ---- synthetic inputs (seeded, reproducible) ---- */
 
 /* Fills rows x cols uint8 values (row pitch `pitch` bytes) with

@@ -1,0 +1,226 @@
+"""Batched persistent launches (dd_batch_*) against the oracle and against
+one launch per job.
+
+A batch runs every tile of every job with the device function and the
+parameters of that job's own launch, so its results must be bit-identical to
+the per-job launches, and, through them, to the oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from dedispersion_amd import DD_SINGLE_STEP, DD_STEP_ONE, DD_STEP_TWO, Batch, Conf, DDError, Plan, workloads
+from dedispersion_amd.beams import BeamSet
+from tests.conftest import host_threads
+from tests.gpu_helpers import compare_rows, make_input, stream_of, zapped_flags
+
+pytestmark = pytest.mark.gpu
+
+STAGED_SMALL = Conf(64, 4, 4, 2, 4, 1)  # 4 waves, 256 samples x 8 DMs per tile, staged
+STAGED_STEP_TWO = Conf(64, 2, 8, 4, 3, 1)
+
+
+def tiny_two_step_workloads():
+    """Uneven jobs: DMs, samples, coarse DMs, beams and zapped channels differ."""
+    shapes = [
+        dict(channels=96, samples=700, dms=12, step=0.9, subbands=8, coarse=9, coarse_step=4.5, beams=2, sbeams=3,
+             zapped=(0, 11, 12, 50, 95), seed=31),
+        dict(channels=96, samples=333, dms=5, step=1.3, subbands=8, coarse=3, coarse_step=6.0, beams=1, sbeams=1,
+             zapped=(), seed=32),
+        dict(channels=64, samples=1100, dms=20, step=0.5, subbands=4, coarse=6, coarse_step=9.0, beams=1, sbeams=2,
+             zapped=tuple(range(16, 32)), seed=33),
+        dict(channels=96, samples=256, dms=8, step=0.7, subbands=8, coarse=1, coarse_step=0.0, beams=1, sbeams=1,
+             zapped=(47,), seed=34),
+    ]
+    return [workloads.tiny(min_freq=1300.0, bandwidth=1.0, **s) for s in shapes]
+
+
+class Jobs:
+    """Plans, device inputs and outputs, and oracle results of a job list."""
+
+    def __init__(self, torch, ws, two_step=True, padding=0):
+        self.torch = torch
+        self.ws = ws
+        self.plans, self.flags, self.inputs, self.mids, self.outs, self.want_mid, self.want_out = ([] for _ in range(7))
+        for w in ws:
+            plan = Plan(w.obs, padding)
+            flags = zapped_flags(plan, w.zapped)
+            plan.set_zapped_channels(flags)
+            z = plan.sizes
+            orc = oracle.Oracle(w.obs, padding, threads=host_threads())
+            if two_step:
+                d_in = make_input(torch, plan, w, z.sub_in_pitch, z.nr_dispersed_samples_subbanding, w.seed)
+                mid = orc.step_one(d_in.cpu().numpy(), flags)
+                self.want_mid.append(mid)
+                self.want_out.append(orc.step_two(mid.ravel()))
+                self.mids.append(torch.full((z.intermediate_elements,), -1.0, dtype=torch.float32, device="cuda"))
+                self.outs.append(torch.full((z.sub_output_elements,), -1.0, dtype=torch.float32, device="cuda"))
+            else:
+                d_in = make_input(torch, plan, w, z.in_pitch, z.nr_dispersed_samples, w.seed)
+                self.want_out.append(orc.dedispersion(d_in.cpu().numpy(), flags))
+                self.mids.append(None)
+                self.outs.append(torch.full((z.output_elements,), -1.0, dtype=torch.float32, device="cuda"))
+            self.plans.append(plan)
+            self.flags.append(flags)
+            self.inputs.append(d_in)
+
+    def reset(self):
+        for m in self.mids:
+            if m is not None:
+                m.fill_(-1.0)
+        for o in self.outs:
+            o.fill_(-1.0)
+        self.torch.cuda.synchronize()
+
+    def check(self, what):
+        self.torch.cuda.synchronize()
+        for j, (w, plan) in enumerate(zip(self.ws, self.plans)):
+            z = plan.sizes
+            if self.mids[j] is not None:
+                want = self.want_mid[j]
+                got = self.mids[j].cpu().numpy().reshape(want.shape)
+                compare_rows(got, want, z.nr_samples_subbanding, f"{what} job {j} step one")
+                want = self.want_out[j]
+                got = self.outs[j].cpu().numpy().reshape(want.shape)
+                compare_rows(got, want, w.obs.nr_samples_per_batch, f"{what} job {j} step two")
+            else:
+                want = self.want_out[j]
+                got = self.outs[j].cpu().numpy().reshape(want.shape)
+                compare_rows(got, want, z.nr_samples, f"{what} job {j} single step")
+
+
+def test_batch_two_step_uneven_jobs(torch_cuda):
+    jobs = Jobs(torch_cuda, tiny_two_step_workloads())
+    with Batch() as b:
+        b.set_conf(DD_STEP_ONE, STAGED_SMALL)
+        b.set_conf(DD_STEP_TWO, STAGED_STEP_TWO)
+        for plan, d_in, d_mid, d_out in zip(jobs.plans, jobs.inputs, jobs.mids, jobs.outs):
+            b.add_two_step(plan, d_in.data_ptr(), d_mid.data_ptr(), d_out.data_ptr())
+        info = b.info()
+        assert info["jobs"] == 4 and info["tiles"][0] == 0 and info["tiles"][1] > 0 and info["tiles"][2] > 0
+        want_units = sum(w.obs.nr_synthesized_beams * w.obs.nr_dms_subbanding * w.obs.nr_dms *
+                         w.obs.nr_samples_per_batch for w in jobs.ws)
+        assert info["units"] == want_units
+        b.run(use_graph=False, stream=stream_of(torch_cuda))
+        jobs.check("persistent")
+        jobs.reset()
+        b.run_per_job(stream=stream_of(torch_cuda))
+        jobs.check("per job")
+
+
+def test_batch_graph_replays(torch_cuda):
+    """The captured graph (queue reset kernel + persistent launches) gives the
+    eager result on every replay, with other device work between replays."""
+    torch = torch_cuda
+    jobs = Jobs(torch, tiny_two_step_workloads()[:3])
+    with Batch() as b:
+        b.set_conf(DD_STEP_ONE, STAGED_SMALL)
+        b.set_conf(DD_STEP_TWO, STAGED_STEP_TWO)
+        for plan, d_in, d_mid, d_out in zip(jobs.plans, jobs.inputs, jobs.mids, jobs.outs):
+            b.add_two_step(plan, d_in.data_ptr(), d_mid.data_ptr(), d_out.data_ptr())
+        scratch = torch.arange(1 << 20, device="cuda", dtype=torch.float32)
+        for replay in range(4):
+            jobs.reset()
+            b.run(use_graph=True, stream=stream_of(torch))
+            _ = (scratch * 3.0).sum().item()  # eager work and a synchronization between replays
+            jobs.check(f"graph replay {replay}")
+
+
+def test_batch_single_step_jobs(torch_cuda):
+    ws = [
+        workloads.tiny(channels=300, samples=500, dms=9, step=1.1, zapped=(3, 299), seed=41, min_freq=1300.0,
+                       bandwidth=1.0),
+        workloads.tiny(channels=40, samples=1000, dms=30, step=0.3, beams=2, sbeams=3, seed=42, min_freq=1300.0,
+                       bandwidth=1.0),
+        workloads.tiny(channels=16, samples=64, dms=8, step=2.0, seed=43),
+    ]
+    jobs = Jobs(torch_cuda, ws, two_step=False)
+    with Batch() as b:
+        b.set_conf(DD_SINGLE_STEP, Conf(64, 2, 8, 2, 4, 1))
+        for plan, d_in, d_out in zip(jobs.plans, jobs.inputs, jobs.outs):
+            b.add_single_step(plan, d_in.data_ptr(), d_out.data_ptr())
+        b.run(use_graph=True, stream=stream_of(torch_cuda))
+        jobs.check("single-step batch")
+
+
+def test_batch_mixed_single_and_two_step(torch_cuda):
+    two = Jobs(torch_cuda, tiny_two_step_workloads()[1:3])
+    one = Jobs(torch_cuda, [workloads.tiny(channels=48, samples=300, dms=10, step=0.8, seed=44, min_freq=1300.0,
+                                           bandwidth=1.0)], two_step=False)
+    with Batch() as b:
+        b.set_conf(DD_SINGLE_STEP, STAGED_SMALL)
+        b.set_conf(DD_STEP_ONE, STAGED_SMALL)
+        b.set_conf(DD_STEP_TWO, STAGED_STEP_TWO)
+        b.add_two_step(two.plans[0], two.inputs[0].data_ptr(), two.mids[0].data_ptr(), two.outs[0].data_ptr())
+        b.add_single_step(one.plans[0], one.inputs[0].data_ptr(), one.outs[0].data_ptr())
+        b.add_two_step(two.plans[1], two.inputs[1].data_ptr(), two.mids[1].data_ptr(), two.outs[1].data_ptr())
+        b.run(stream=stream_of(torch_cuda))
+        two.check("mixed batch")
+        one.check("mixed batch")
+
+
+def test_batch_rejects_unstaged_configuration(torch_cuda):
+    jobs = Jobs(torch_cuda, tiny_two_step_workloads()[:1])
+    with Batch() as b:
+        b.add_two_step(jobs.plans[0], jobs.inputs[0].data_ptr(), jobs.mids[0].data_ptr(), jobs.outs[0].data_ptr())
+        b.set_conf(DD_STEP_ONE, Conf(64, 4, 4, 2, 4, 0))  # global-memory kernel: not batchable
+        with pytest.raises(DDError, match="staged"):
+            b.run(stream=stream_of(torch_cuda))
+        b.set_conf(DD_STEP_ONE, Conf(64, 2, 4, 2, 8, 1))  # unroll > 2 * threads_d1
+        with pytest.raises(DDError, match="staged"):
+            b.run(stream=stream_of(torch_cuda))
+        b.set_conf(DD_STEP_ONE, STAGED_SMALL)
+        b.run(stream=stream_of(torch_cuda))
+        jobs.check("after rejected configurations")
+
+
+def test_batched_beams_40_match_per_beam_launches(torch_cuda):
+    """The 40-beam uneven workload: the graph-replayed persistent batch equals
+    one launch per beam and step, bit for bit, on every beam; a few rows of
+    two beams are checked against the oracle directly."""
+    torch = torch_cuda
+    ws = workloads.batched_beams()
+    bs = BeamSet(torch, ws)
+    try:
+        info = bs.info()
+        assert info["jobs"] == 40
+        bs.run_per_job()
+        torch.cuda.synchronize()
+        ref_mid = [m.clone() for m in bs.mids]
+        ref_out = [o.clone() for o in bs.outs]
+        for m in bs.mids:
+            m.fill_(-7.0)
+        for o in bs.outs:
+            o.fill_(-7.0)
+        bs.run(use_graph=True)
+        bs.run(use_graph=True)
+        torch.cuda.synchronize()
+        for j, plan in enumerate(bs.plans):
+            z = plan.sizes
+            o = plan.obs
+            got = bs.outs[j].view(o.nr_dms_subbanding * o.nr_dms, z.out_pitch)[:, : z.nr_samples]
+            want = ref_out[j].view(o.nr_dms_subbanding * o.nr_dms, z.out_pitch)[:, : z.nr_samples]
+            assert torch.equal(got, want), f"beam {j} ({ws[j].name}): batch differs from per-beam launches"
+            gm = bs.mids[j].view(-1, z.sub_out_pitch)[:, : z.nr_samples_subbanding]
+            wm = ref_mid[j].view(-1, z.sub_out_pitch)[:, : z.nr_samples_subbanding]
+            assert torch.equal(gm, wm), f"beam {j}: intermediate differs"
+        del ref_mid, ref_out
+        for j in (0, 7):
+            plan = bs.plans[j]
+            w = ws[j]
+            o, z = w.obs, plan.sizes
+            orc = oracle.Oracle(o, bs.padding, threads=host_threads())
+            flags = zapped_flags(plan, w.zapped)
+            inp = bs.inputs[j].cpu().numpy()
+            mid = bs.mids[j].cpu().numpy()
+            c = o.nr_dms_subbanding - 2
+            want = orc.step_one(inp, flags, rows=(c, c + 2))
+            got = mid.reshape(1, o.nr_dms_subbanding, o.nr_subbands, z.sub_out_pitch)
+            compare_rows(got[:, c:c + 2], want[:, c:c + 2], z.nr_samples_subbanding, f"beam {j} step one")
+            r0, r1 = c * o.nr_dms, (c + 1) * o.nr_dms
+            want = orc.step_two(mid, rows=(r0, r1))
+            got = bs.outs[j].cpu().numpy().reshape(1, o.nr_dms_subbanding * o.nr_dms, z.out_pitch)
+            compare_rows(got[:, r0:r1], want[:, r0:r1], o.nr_samples_per_batch, f"beam {j} step two")
+    finally:
+        bs.close()
+        torch.cuda.empty_cache()

@@ -1,11 +1,20 @@
 """Benchmark of the MI355X dedispersion engine (one JSON line on stdout).
 
 A step is one pass of the hot path over one batch with the input already
-resident in HBM.  The workload and metric are DM-trial·samples per second:
-one unit is one element of the final dedispersed output (beam x DM trial x
-sample).  The dominant kernel's average duration is measured with HIP events
-recorded on the stream the kernels run on, and turned into an HBM-roofline
-fraction from its algorithmic bytes.
+resident in HBM.  The metric is DM-trial·samples per second: one unit is one
+element of the final dedispersed output (beam x DM trial x sample).
+
+Default workload: 40 synthesized beams of 1536 channels with seeded uneven
+DM counts {512..4096} and sample counts {8192..32768}, two-step subband
+dedispersion (32 subbands, coarse DMs of 6.4, 32 fine DMs of 0.2), run as a
+batch: one persistent launch for step one of every beam and one for step two,
+replayed from a HIP graph.  The same work with one launch per beam and step
+(the reference's launch pattern) is timed beside it.
+
+Each persistent kernel's average duration is measured with HIP events on the
+stream it runs on and turned into an HBM-roofline fraction from its
+algorithmic bytes (input read once, intermediate written and read once,
+output written once, padded rows).
 """
 from __future__ import annotations
 
@@ -30,11 +39,31 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="arts-two-step",
-                    choices=["arts-two-step", "arts-single"])
+    ap.add_argument("--workload", default="batched-beams",
+                    choices=["batched-beams", "arts-two-step", "arts-single"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run --steps plain device-to-device copies of BYTES (for scaling HBM counters)")
     return ap.parse_args()
+
+
+def copy_calibration(args, torch):
+    """Plain device copies of a known size: the FETCH_SIZE / WRITE_SIZE a
+    profiler reports for them calibrate the counters against bytes moved."""
+    n = args.copy_calibration
+    src = torch.randint(0, 255, (n,), dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    a, b = events(torch, 2)
+    a.record()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / args.steps
+    print(json.dumps({"copy_bytes": n, "steps": args.steps, "ms_per_copy": ms,
+                      "GB_per_s_read_plus_write": 2 * n / (ms / 1e3) / 1e9}), flush=True)
 
 
 class EventTimer:
@@ -145,14 +174,115 @@ def cpu_baseline(w, mode: str, seconds: float):
             "seconds": round(dt, 3)}
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU per process; multi-GPU runs are not part of this build",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
-    import torch
+def measured_traffic(workload: str, kernel: str, conf: str):
+    """HBM bytes per launch of `kernel` under `conf` on `workload`, from the
+    calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE runs committed under
+    profiles/ (None when no run matches)."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f).get(workload, {}).get(kernel)
+    except (OSError, ValueError):
+        return None
+    if not rec or rec.get("conf") != conf:
+        return None
+    return rec["traffic_bytes"]
 
+
+def events(torch, n):
+    return [torch.cuda.Event(enable_timing=True) for _ in range(n)]
+
+
+def run_batched(args, torch):
+    from dedispersion_amd import DD_STEP_ONE, DD_STEP_TWO, workloads
+    from dedispersion_amd.beams import BeamSet
+
+    ws = workloads.batched_beams()
+    bs = BeamSet(torch, ws)
+    info = bs.info()
+    units = info["units"]
+    log(f"batched beams: {info}")
+    for _ in range(args.warmup):
+        bs.run(use_graph=True)
+        bs.run_per_job()
+    torch.cuda.synchronize()
+
+    # timed region: K graph replays of the whole batch
+    start, stop = events(torch, 2)
+    t0 = time.perf_counter()
+    start.record()
+    for _ in range(args.steps):
+        bs.run(use_graph=True)
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = start.elapsed_time(stop) / args.steps
+
+    # per-kernel durations: the two persistent launches, events around each
+    names = {DD_STEP_ONE: "step_one_batch", DD_STEP_TWO: "step_two_batch"}
+    pairs = {m: [] for m in names}
+    for _ in range(args.steps):
+        for m in names:
+            a, b = events(torch, 2)
+            a.record()
+            bs.run_mode(m)
+            b.record()
+            pairs[m].append((a, b))
+    torch.cuda.synchronize()
+    avg = {m: sum(a.elapsed_time(b) for a, b in v) / len(v) for m, v in pairs.items()}
+
+    # the reference's launch pattern: one launch per beam and step
+    a, b = events(torch, 2)
+    a.record()
+    for _ in range(args.steps):
+        bs.run_per_job()
+    b.record()
+    torch.cuda.synchronize()
+    ms_plain = a.elapsed_time(b) / args.steps
+
+    step_bytes = bs.step_bytes()
+    dominant = max(avg, key=avg.get)
+    achieved = step_bytes[dominant] / (avg[dominant] / 1e3)
+    total_bytes = bs.bytes_moved()
+    value = units / (ms / 1e3)
+    dms = sorted({w.obs.nr_dms_subbanding * w.obs.nr_dms for w in ws})
+    workload = f"batched-beams-{len(ws)}x{ws[0].obs.nr_channels}ch-two-step"
+    confs = {names[m]: bs.batch.get_conf(m).print() for m in names}
+    samples = sorted({w.obs.nr_samples_per_batch for w in ws})
+    line = {
+        "metric": "DM-trial*samples/s",
+        "value": value,
+        "unit": "DM-trial*samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8/f32",
+        "data": "synthetic (seeded Gaussian noise N(128,16) clipped to uint8 + 8 dispersed pulses per beam, "
+                "5% zapped channels per beam)",
+        "config": {"workload": workload, "beams": len(ws),
+                   "channels": ws[0].obs.nr_channels, "subbands": ws[0].obs.nr_subbands, "dms_per_beam": dms,
+                   "samples_per_beam": samples, "units_per_step": units, "padding": 128, "launch": "persistent+graph",
+                   "tiles": info["tiles"], "workgroups": info["workgroups"],
+                   "conf": confs},
+        "plain_per_beam": {"value": units / (ms_plain / 1e3), "ms_per_step": ms_plain,
+                           "launches_per_step": 2 * len(ws)},
+        "batched_vs_plain": ms_plain / ms,
+        "hbm_roofline_frac_path": total_bytes / (ms / 1e3) / HBM_PEAK,
+        "kernels_ms": {names[m]: v for m, v in avg.items()},
+        "roofline": {"bound": "hbm", "kernel": names[dominant], "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": measured_traffic(workload, names[dominant], confs[names[dominant]]),
+                     "algorithmic_bytes": step_bytes[dominant]},
+        "wall_s": wall,
+    }
+    return line, ws[0], "two-step"
+
+
+def run_single_plan(args, torch):
     from dedispersion_amd import workloads
 
     if args.workload == "arts-two-step":
@@ -170,8 +300,7 @@ def main():
         for fn, _ in kernels.values():
             fn()
     torch.cuda.synchronize()
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
+    start, stop = events(torch, 2)
     t0 = time.perf_counter()
     start.record()
     for _ in range(args.steps):
@@ -198,7 +327,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u8-in/f32-acc",
+        "dtype": "u8/f32" if mode == "two-step" else "u8",
         "data": "synthetic (seeded Gaussian noise N(128,16) + 8 dispersed pulses, 5% zapped channels)",
         "config": {"workload": w.name, "mode": mode, "beams": w.obs.nr_synthesized_beams,
                    "channels": w.obs.nr_channels, "samples": w.obs.nr_samples_per_batch,
@@ -209,6 +338,24 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None},
         "wall_s": wall,
     }
+    return line, w, mode
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU per process; multi-GPU runs are not part of this build",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+
+    if args.copy_calibration:
+        copy_calibration(args, torch)
+        return
+    if args.workload == "batched-beams":
+        line, w, mode = run_batched(args, torch)
+    else:
+        line, w, mode = run_single_plan(args, torch)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(w, mode, args.cpu_seconds)

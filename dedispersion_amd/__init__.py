@@ -265,6 +265,9 @@ class Batch:
     def run(self, use_graph: bool = False, stream: int = 0) -> None:
         check(self.lib.dd_batch_run(self.handle, 1 if use_graph else 0, stream or None))
 
+    def run_mode(self, mode: int, stream: int = 0) -> None:
+        check(self.lib.dd_batch_run_mode(self.handle, mode, stream or None))
+
     def run_per_job(self, stream: int = 0) -> None:
         check(self.lib.dd_batch_run_per_job(self.handle, stream or None))
 

@@ -126,6 +126,7 @@ PROTOTYPES = {
     "dd_batch_set_conf": (c_int, [c_void_p, c_int, POINTER(DDConf)]),
     "dd_batch_get_conf": (c_int, [c_void_p, c_int, POINTER(DDConf)]),
     "dd_batch_run": (c_int, [c_void_p, c_int, c_void_p]),
+    "dd_batch_run_mode": (c_int, [c_void_p, c_int, c_void_p]),
     "dd_batch_run_per_job": (c_int, [c_void_p, c_void_p]),
     "dd_batch_get_info": (c_int, [c_void_p, POINTER(DDBatchInfo)]),
     "dd_synth_gaussian_u8": (c_int, [c_void_p, c_uint64, c_uint64, c_uint64, c_float, c_float, c_uint64, c_void_p]),

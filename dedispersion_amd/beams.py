@@ -64,6 +64,20 @@ class BeamSet:
     def run(self, use_graph: bool = True) -> None:
         self.batch.run(use_graph, self.stream)
 
+    def run_mode(self, mode: int) -> None:
+        self.batch.run_mode(mode, self.stream)
+
+    def step_bytes(self) -> dict:
+        """Compulsory HBM bytes of each step over the whole set (padded rows):
+        step one reads the input and writes the intermediate, step two reads
+        the intermediate and writes the output."""
+        one = two = 0
+        for plan in self.plans:
+            z = plan.sizes
+            one += z.sub_input_elements + 4 * z.intermediate_elements
+            two += 4 * z.intermediate_elements + 4 * z.sub_output_elements
+        return {DD_STEP_ONE: one, DD_STEP_TWO: two}
+
     def run_per_job(self) -> None:
         self.batch.run_per_job(self.stream)
 

@@ -211,14 +211,22 @@ dd_status build(dd_batch *b) {
   return DD_OK;
 }
 
+// Clears the claim counters of kind m and launches its persistent kernel.
+dd_status enqueue_kind(dd_batch *b, int m, hipStream_t s) {
+  KindQueue &k = b->q[m];
+  if (k.n_tiles == 0) return DD_OK;
+  uint32_t *heads = b->heads.ptr + m * kQueues * kHeadStride;
+  DD_HIP(launch_clear_words(heads, kQueues * kHeadStride, s));
+  TileQueues tq{k.d_tiles.ptr, k.d_begin.ptr, heads};
+  if (m == DD_STEP_TWO) DD_HIP(launch_rowsum_f32_batch(k.d_f32.ptr, tq, k.g, k.workgroups, s));
+  else DD_HIP(launch_rowsum_u8_batch(k.d_u8.ptr, tq, k.g, k.flush, k.workgroups, s));
+  return DD_OK;
+}
+
 dd_status enqueue(dd_batch *b, hipStream_t s) {
-  DD_HIP(launch_clear_words(b->heads.ptr, (uint32_t)b->heads.count, s));
   for (int m = 0; m < 3; ++m) {
-    KindQueue &k = b->q[m];
-    if (k.n_tiles == 0) continue;
-    TileQueues tq{k.d_tiles.ptr, k.d_begin.ptr, b->heads.ptr + m * kQueues * kHeadStride};
-    if (m == DD_STEP_TWO) DD_HIP(launch_rowsum_f32_batch(k.d_f32.ptr, tq, k.g, k.workgroups, s));
-    else DD_HIP(launch_rowsum_u8_batch(k.d_u8.ptr, tq, k.g, k.flush, k.workgroups, s));
+    dd_status st = enqueue_kind(b, m, s);
+    if (st != DD_OK) return st;
   }
   return DD_OK;
 }
@@ -299,6 +307,13 @@ dd_status dd_batch_run(dd_batch *batch, int use_graph, void *stream) {
   }
   DD_HIP(hipGraphLaunch(batch->exec, (hipStream_t)stream));
   return DD_OK;
+}
+
+dd_status dd_batch_run_mode(dd_batch *batch, dd_mode mode, void *stream) {
+  if (!batch || (int)mode < 0 || (int)mode > 2) return fail(DD_ERR_INVALID_ARGUMENT, "invalid argument");
+  if (batch->jobs.empty()) return DD_OK;
+  dd_status s = build(batch);
+  return s != DD_OK ? s : enqueue_kind(batch, (int)mode, (hipStream_t)stream);
 }
 
 dd_status dd_batch_run_per_job(dd_batch *batch, void *stream) {

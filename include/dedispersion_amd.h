@@ -232,6 +232,10 @@ dd_status dd_batch_get_conf(const dd_batch *batch, dd_mode mode, dd_conf *conf);
  * batch on `stream`.  use_graph != 0: the launches are captured once into a
  * HIP graph that later runs replay; the graph is rebuilt after any change. */
 dd_status dd_batch_run(dd_batch *batch, int use_graph, void *stream);
+/* Enqueues the persistent launch of one kernel kind only (no graph): for
+ * timing a kind, or running step one and step two of a batch apart.  Step
+ * two reads what step one of the same batch wrote: run DD_STEP_ONE first. */
+dd_status dd_batch_run_mode(dd_batch *batch, dd_mode mode, void *stream);
 /* The same work as one ordinary launch per job and kernel kind, in job
  * order (the reference's launch pattern), with the batch configurations. */
 dd_status dd_batch_run_per_job(dd_batch *batch, void *stream);

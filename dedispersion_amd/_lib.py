@@ -96,6 +96,34 @@ class DDBatchInfo(ctypes.Structure):
     ]
 
 
+class DDStreamInfo(ctypes.Structure):
+    _fields_ = [
+        ("slots", c_uint32),
+        ("hip_streams", c_uint32),
+        ("chunk_samples", c_uint64),
+        ("history_samples", c_uint64),
+        ("slot_bytes", c_uint64),
+        ("history_bytes", c_uint64),
+        ("queued", c_uint32),
+        ("in_flight", c_uint32),
+        ("submitted", c_uint64),
+        ("completed", c_uint64),
+    ]
+
+
+class DDStreamEvent(ctypes.Structure):
+    _fields_ = [
+        ("tag", c_uint64),
+        ("beam", c_uint32),
+        ("slot", c_uint32),
+        ("chunk", c_uint64),
+        ("first_sample", ctypes.c_int64),
+        ("output", c_void_p),
+        ("intermediate", c_void_p),
+        ("device_seconds", ctypes.c_double),
+    ]
+
+
 # name -> (restype, argtypes); every exported symbol of include/dedispersion_amd.h
 PROTOTYPES = {
     "dd_last_error": (c_char_p, []),
@@ -129,6 +157,23 @@ PROTOTYPES = {
     "dd_batch_run_mode": (c_int, [c_void_p, c_int, c_void_p]),
     "dd_batch_run_per_job": (c_int, [c_void_p, c_void_p]),
     "dd_batch_get_info": (c_int, [c_void_p, POINTER(DDBatchInfo)]),
+    "dd_stream_create": (c_int, [POINTER(c_void_p), c_void_p, c_uint32, c_uint64, c_uint32, c_uint32]),
+    "dd_stream_destroy": (c_int, [c_void_p]),
+    "dd_stream_get_info": (c_int, [c_void_p, POINTER(DDStreamInfo)]),
+    "dd_stream_submit": (c_int, [c_void_p, c_uint32, c_void_p, c_uint64, c_uint64]),
+    "dd_stream_progress": (c_int, [c_void_p, POINTER(DDStreamEvent), c_uint32, POINTER(c_uint32)]),
+    "dd_stream_release": (c_int, [c_void_p, c_uint32]),
+    "dd_stream_drain": (c_int, [c_void_p]),
+    "dd_mlp_create": (c_int, [POINTER(c_void_p), c_uint32, c_uint32, c_uint32]),
+    "dd_mlp_destroy": (c_int, [c_void_p]),
+    "dd_mlp_param_count": (c_int, [c_void_p, POINTER(c_uint32)]),
+    "dd_mlp_set_params": (c_int, [c_void_p, POINTER(c_float)]),
+    "dd_mlp_get_params": (c_int, [c_void_p, POINTER(c_float)]),
+    "dd_mlp_init": (c_int, [c_void_p, c_uint64]),
+    "dd_mlp_forward": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p, c_void_p]),
+    "dd_mlp_loss_grad": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p]),
+    "dd_mlp_adam_step": (c_int, [c_void_p, c_void_p, c_float, c_void_p]),
+    "dd_mlp_train": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_float, POINTER(c_float), c_void_p]),
     "dd_synth_gaussian_u8": (c_int, [c_void_p, c_uint64, c_uint64, c_uint64, c_float, c_float, c_uint64, c_void_p]),
     "dd_synth_uniform_u8": (c_int, [c_void_p, c_uint64, c_uint64, c_uint64, c_uint64, c_void_p]),
     "dd_synth_inject_pulse": (

@@ -61,6 +61,7 @@ dd_status build_integer(dd_plan *p, ShiftTable &t, const float *d_table, uint32_
                         float first, float step) {
   t.rows = rows;
   t.dms = dms;
+  t.spread_cache.clear();
   t.pitch = roundup(std::max(dms, 1u), 64) + 64;
   DD_HIP(t.dev.allocate((size_t)rows * t.pitch));
   if (dms == 0 || rows == 0) {

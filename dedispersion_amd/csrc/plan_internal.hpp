@@ -54,16 +54,21 @@ struct ShiftTable {
       for (uint32_t d = 0; d < dms; ++d) m = std::max(m, at(r, d));
     return m;
   }
-  // Largest shift spread inside any DM tile of `tile` consecutive DMs.
+  // Largest shift spread inside any DM tile of `tile` consecutive DMs
+  // (cached per tile size; the cache is cleared when the table changes).
   uint32_t max_spread(uint32_t tile) const {
+    for (auto &c : spread_cache)
+      if (c.first == tile) return c.second;
     uint32_t m = 0;
     for (uint32_t r = 0; r < rows; ++r)
       for (uint32_t d0 = 0; d0 < dms; d0 += tile) {
         uint32_t a = at(r, d0), b = at(r, std::min(d0 + tile - 1, pitch - 1));
         m = std::max(m, b - a);
       }
+    spread_cache.emplace_back(tile, m);
     return m;
   }
+  mutable std::vector<std::pair<uint32_t, uint32_t>> spread_cache;
 };
 
 } // namespace ddamd

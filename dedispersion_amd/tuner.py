@@ -192,38 +192,80 @@ def write_tuned_file(path: str, entries: List[str]) -> None:
             f.write(e + "\n")
 
 
+def _beam_shapes():
+    """Layouts of the 40-beam workload (two-step, coarse DMs x 32 fine DMs of
+    0.2, samples per beam), held out of the learned tuner's training set."""
+    return [wl.arts_two_step(dc, s, seed=500 + i) for i, (dc, s) in
+            enumerate(((16, 8192), (32, 16384), (128, 32768), (64, 8192)))]
+
+
 WORKLOADS = {
-    "plain-128": (wl.plain_128, ["single_step"]),
-    "arts-single": (wl.arts_single, ["single_step"]),
-    "arts-two-step": (wl.arts_two_step, ["step_one", "step_two"]),
+    "plain-128": (lambda: [wl.plain_128()], ["single_step"]),
+    "arts-single": (lambda: [wl.arts_single()], ["single_step"]),
+    "arts-two-step": (lambda: [wl.arts_two_step()], ["step_one", "step_two"]),
+    "beam-shapes": (_beam_shapes, ["step_one", "step_two"]),
 }
 
 
+def learned_tune(bench: Bench, model, budget: float, iterations: int) -> Dict:
+    """Times the model's best-predicted fraction `budget` of the space."""
+    from . import learned
+
+    def time_fn(c):
+        t = bench.time(Conf(**c), iterations=iterations)
+        return None if t is None else t["ms"]
+
+    return learned.pick(model, bench.layout, [asdict(c) for c in enumerate_space()], time_fn, budget)
+
+
 def main(argv=None):
-    ap = argparse.ArgumentParser(description="exhaustive HIP kernel tuning sweep")
+    ap = argparse.ArgumentParser(description="HIP kernel tuning: exhaustive sweep or learned pruning")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="arts-two-step")
     ap.add_argument("--modes", default=None, help="comma list of single_step,step_one,step_two")
     ap.add_argument("--records", default=None, help="JSON-lines file to append to (resumable)")
     ap.add_argument("--tuned", default=None, help="write the best configurations in the reference format")
     ap.add_argument("--iterations", type=int, default=5)
     ap.add_argument("--device", default="MI355X")
+    ap.add_argument("--learned", action="store_true",
+                    help="train the cost model on the recorded sweeps and time only the best-predicted fraction")
+    ap.add_argument("--budget", type=float, default=0.1, help="fraction of the space timed with --learned")
+    ap.add_argument("--train-records", default=None, help="sweep records for --learned (default: the packaged ones)")
     args = ap.parse_args(argv)
     import torch
 
     make, modes = WORKLOADS[args.workload]
     if args.modes:
         modes = args.modes.split(",")
-    w = make()
+    model = None
+    if args.learned:
+        from . import learned
+
+        records = learned.load_records(args.train_records or learned.RECORDS_PATH)
+        model = learned.CostModel()
+        losses = model.fit(records)
+        print(f"cost model trained on {len(records)} records: loss {losses[0]:.3f} -> {losses[-1]:.4f}",
+              file=sys.stderr, flush=True)
     lines = []
-    for m in modes:
-        bench = Bench(w, MODES[m], torch)
-        t0 = time.time()
-        recs = sweep(bench, enumerate_space(), args.records, iterations=args.iterations, device=args.device)
-        b = best_of(recs)
-        print(f"{w.name} {m}: {len(recs)} configurations in {time.time() - t0:.1f}s; best "
-              f"{b['conf'] if b else None} {b['ms'] if b else None} ms", file=sys.stderr, flush=True)
-        if b:
-            lines.append(tuned_conf_line(args.device, bench.layout["dms"], b["conf"]))
+    for w in make():
+        for m in modes:
+            bench = Bench(w, MODES[m], torch)
+            t0 = time.time()
+            if model is not None:
+                b = learned_tune(bench, model, args.budget, args.iterations)
+                print(f"{w.name} {m}: timed {b['timed']} of {b['candidates']} in {time.time() - t0:.1f}s; best "
+                      f"{b['conf']} {b['ms']} ms", file=sys.stderr, flush=True)
+                conf = b["conf"]
+            else:
+                recs = sweep(bench, enumerate_space(), args.records, iterations=args.iterations, device=args.device)
+                b = best_of(recs)
+                print(f"{w.name} {m}: {len(recs)} configurations in {time.time() - t0:.1f}s; best "
+                      f"{b['conf'] if b else None} {b['ms'] if b else None} ms", file=sys.stderr, flush=True)
+                conf = b["conf"] if b else None
+            if conf:
+                conf = {k: v for k, v in conf.items() if k in Conf.__dataclass_fields__}
+                lines.append(tuned_conf_line(args.device, bench.layout["dms"], conf))
+            del bench
+            torch.cuda.empty_cache()
     if args.tuned:
         write_tuned_file(args.tuned, lines)
 

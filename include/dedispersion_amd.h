@@ -241,6 +241,92 @@ dd_status dd_batch_run_mode(dd_batch *batch, dd_mode mode, void *stream);
 dd_status dd_batch_run_per_job(dd_batch *batch, void *stream);
 dd_status dd_batch_get_info(dd_batch *batch, dd_batch_info *info);
 
+/* ---- streaming: continuous two-step dedispersion of chunked beams ----
+ *
+ * No counterpart in the reference (its driver processes one batch,
+ * src/DedispersionTest.cpp:129-416).  `plan` is a two-step plan of ONE beam
+ * whose nr_samples_per_batch S is the chunk length; every beam of the stream
+ * shares it.  Chunk k of a beam is [C][S] uint8 host memory (row pitch
+ * host_pitch bytes; pinned memory makes the copy asynchronous) and must stay
+ * valid until its completion is reported.  The engine keeps the last
+ * H = nr_dispersed_samples_subbanding - S samples of each beam on the device
+ * and dedisperses [history | chunk]: the output rows of chunk k hold the
+ * beam's samples [k S - H, k S - H + S) (zero history before the first
+ * chunk).  Requests are admitted into slots (input, intermediate and output
+ * buffers of the plan) while the HBM budget allows, in submission order,
+ * chunks of one beam in order; each admitted request runs on one of
+ * `hip_streams` streams, so requests admitted while others run overlap.  An
+ * engine is used by one host thread at a time. */
+typedef struct dd_stream dd_stream;
+
+typedef struct dd_stream_info {
+  uint32_t slots;            /* requests that can be in flight at once */
+  uint32_t hip_streams;
+  uint64_t chunk_samples;    /* S */
+  uint64_t history_samples;  /* H */
+  uint64_t slot_bytes;       /* device bytes per slot */
+  uint64_t history_bytes;    /* device bytes of all beam histories */
+  uint32_t queued;           /* submitted, waiting for a slot */
+  uint32_t in_flight;        /* admitted, not released */
+  uint64_t submitted, completed;
+} dd_stream_info;
+
+typedef struct dd_stream_event {
+  uint64_t tag;              /* the caller's tag of the request */
+  uint32_t beam;
+  uint32_t slot;             /* pass to dd_stream_release once the output is consumed */
+  uint64_t chunk;            /* index of the chunk within its beam */
+  int64_t first_sample;      /* beam sample of output column 0: chunk * S - H */
+  float *output;             /* device, step-two output [Dc * Df][out_pitch] of the plan */
+  float *intermediate;       /* device, step-one output of the plan */
+  double device_seconds;     /* admission to observed completion */
+} dd_stream_event;
+
+/* Slots = min(max_slots, (hbm_budget_bytes - history bytes) / slot bytes). */
+dd_status dd_stream_create(dd_stream **stream, dd_plan *plan, uint32_t nr_beams, uint64_t hbm_budget_bytes,
+                           uint32_t max_slots, uint32_t hip_streams);
+dd_status dd_stream_destroy(dd_stream *stream);
+dd_status dd_stream_get_info(const dd_stream *stream, dd_stream_info *info);
+/* Queues the next chunk of `beam` and admits what can run. */
+dd_status dd_stream_submit(dd_stream *stream, uint32_t beam, const uint8_t *host_chunk, uint64_t host_pitch,
+                           uint64_t tag);
+/* Admits what can run and reports up to max_events newly completed requests. */
+dd_status dd_stream_progress(dd_stream *stream, dd_stream_event *events, uint32_t max_events, uint32_t *n_events);
+/* Frees the slot of a reported request (its output is overwritten later). */
+dd_status dd_stream_release(dd_stream *stream, uint32_t slot);
+/* Waits until all admitted work has finished on the device. */
+dd_status dd_stream_drain(dd_stream *stream);
+
+/* ---- cost model of the learned tuner ----
+ *
+ * No counterpart in the reference, whose tuner (src/DedispersionTuning.cpp:
+ * 201-427) times every configuration.  A small FP32 network -- `inputs`
+ * features, `hidden_layers` ReLU layers of `hidden` units, one linear output
+ * -- is trained on the device on timings of that exhaustive sweep and ranks
+ * untimed configurations.  Parameters are one flat array: per layer, the
+ * weights (fan_in x fan_out, row-major, W[i * fan_out + j] joins input i to
+ * unit j) then the biases.  Device pointers are float arrays of n rows. */
+typedef struct dd_mlp dd_mlp;
+dd_status dd_mlp_create(dd_mlp **net, uint32_t inputs, uint32_t hidden, uint32_t hidden_layers);
+dd_status dd_mlp_destroy(dd_mlp *net);
+dd_status dd_mlp_param_count(const dd_mlp *net, uint32_t *count);
+dd_status dd_mlp_set_params(dd_mlp *net, const float *params); /* host array; resets the optimizer */
+dd_status dd_mlp_get_params(const dd_mlp *net, float *params); /* host array */
+/* He-uniform weights from `seed`, zero biases. */
+dd_status dd_mlp_init(dd_mlp *net, uint64_t seed);
+/* d_y[r] = network(d_x[r][0..inputs)) for r < n. */
+dd_status dd_mlp_forward(dd_mlp *net, const float *d_x, uint32_t n, float *d_y, void *stream);
+/* Mean squared error L = (1/n) sum (y_r - t_r)^2 into d_loss[0] and dL/dparams
+ * into d_grad (flat parameter layout). */
+dd_status dd_mlp_loss_grad(dd_mlp *net, const float *d_x, const float *d_target, uint32_t n, float *d_loss,
+                           float *d_grad, void *stream);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with gradient d_grad. */
+dd_status dd_mlp_adam_step(dd_mlp *net, const float *d_grad, float lr, void *stream);
+/* `steps` full-batch Adam steps; the loss before each step goes to the host
+ * array `losses` [steps] when it is not NULL.  Synchronizes the stream. */
+dd_status dd_mlp_train(dd_mlp *net, const float *d_x, const float *d_target, uint32_t n, uint32_t steps, float lr,
+                       float *losses, void *stream);
+
 /* ---- synthetic inputs (seeded, reproducible) ---- */
 
 /* Fills rows x cols uint8 values (row pitch `pitch` bytes) with

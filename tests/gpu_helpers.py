@@ -61,3 +61,22 @@ def compare_rows(got: np.ndarray, want: np.ndarray, valid: int, what: str):
     w = want[..., :valid]
     bad = np.argwhere(g != w)
     assert bad.size == 0, f"{what}: {len(bad)} mismatches, first at {bad[:5].tolist()}: got {g[tuple(bad[0])]} want {w[tuple(bad[0])]}"
+
+
+_HIP = None
+
+
+def device_to_numpy(ptr: int, count: int, dtype) -> np.ndarray:
+    """Copies `count` elements at a raw device address to a new host array
+    (hipMemcpy through the HIP runtime the process already loaded)."""
+    import ctypes
+
+    global _HIP
+    if _HIP is None:
+        _HIP = ctypes.CDLL("libamdhip64.so")
+        _HIP.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        _HIP.hipMemcpy.restype = ctypes.c_int
+    out = np.empty(count, dtype=dtype)
+    rc = _HIP.hipMemcpy(out.ctypes.data, ptr, out.nbytes, 2)  # hipMemcpyDeviceToHost
+    assert rc == 0, f"hipMemcpy failed: {rc}"
+    return out

@@ -1,0 +1,81 @@
+"""The learned tuner: the cost model is trained on the device (dd_mlp_train)
+on the exhaustive-sweep records of the 128-channel, ARTS single-step and
+ARTS two-step layouts (2880 timed configurations), then ranks the
+configurations of layouts it has not seen -- the shapes of the 40-beam
+workload -- and only the best-predicted 10% are timed.  The exhaustive sweep
+is the judge: the pick must be within 5% of its optimum, for at most 10% of
+its timing cost, and must beat the same budget spent without the model."""
+import numpy as np
+import pytest
+
+from dedispersion_amd import DD_STEP_TWO, Conf, learned, tuner, workloads
+
+pytestmark = pytest.mark.gpu
+
+TRAIN_WORKLOADS = ("plain-128ch-128dm-4096s", "arts-1536ch-2048dm-24576s", "arts-two-step-32sb-64x32dm-24576s")
+
+
+@pytest.fixture(scope="module")
+def trained(torch_cuda):
+    recs = learned.load_records()
+    train = [r for r in recs if r["workload"] in TRAIN_WORKLOADS]
+    held_out = {k: v for k, v in learned.by_layout(recs).items() if v[0]["workload"] not in TRAIN_WORKLOADS}
+    assert len(train) == 2880 and len(held_out) == 8
+    model = learned.CostModel(seed=0, members=3)
+    losses = model.fit(train, steps=3000, lr=2e-3)
+    assert losses.shape == (3, 3000)
+    yield model, losses, held_out
+    model.close()
+
+
+def test_training_converges(trained):
+    _, losses, _ = trained
+    assert np.all(np.isfinite(losses))
+    assert np.all(losses[:, -1] < 0.01 * losses[:, 0]), (losses[:, 0], losses[:, -1])
+
+
+def test_picks_within_5_percent_in_10_percent_of_the_sweep(trained):
+    model, _, held_out = trained
+    lines = []
+    for key, recs in held_out.items():
+        r = learned.evaluate_offline(model, recs, budget=0.1)
+        lines.append(f"{recs[0]['workload']} {recs[0]['layout']['mode']}: ratio {r['ratio']:.3f}, "
+                     f"time {r['time_fraction']:.3f}, configurations {r['timed']}/{r['candidates']}")
+        assert r["ratio"] <= 1.05, lines[-1]
+        assert r["time_fraction"] <= 0.10, lines[-1]
+    print("\n".join(lines))
+
+
+def test_gain_over_the_same_budget_without_the_model(trained):
+    """Without the trained network the same 10% budget is spent on random
+    configurations (expected best over 2000 draws) or on the ranking of an
+    untrained network; both land further from the optimum."""
+    model, _, held_out = trained
+    untrained = learned.CostModel(seed=0)
+    try:
+        learned_r, random_r, untrained_r = [], [], []
+        for recs in held_out.values():
+            learned_r.append(learned.evaluate_offline(model, recs, 0.1)["ratio"])
+            random_r.append(learned.random_pick_ratio(recs, 0.1))
+            untrained_r.append(learned.evaluate_offline(untrained, recs, 0.1)["ratio"])
+    finally:
+        untrained.close()
+    print(f"mean ratio to optimum: learned {np.mean(learned_r):.4f}, random {np.mean(random_r):.4f}, "
+          f"untrained {np.mean(untrained_r):.4f}")
+    assert np.mean(learned_r) < np.mean(random_r)
+    assert np.mean(learned_r) < np.mean(untrained_r)
+    assert max(learned_r) <= 1.05
+
+
+def test_live_pick_against_live_exhaustive_sweep(torch_cuda, trained):
+    """On the device, for step two of a held-out 32-coarse-DM x 16384-sample
+    beam: the learned tuner times 72 configurations, the exhaustive sweep all
+    720; the learned pick is within 5% of the sweep's best."""
+    model, _, _ = trained
+    w = workloads.arts_two_step(32, 16384, seed=501)
+    bench = tuner.Bench(w, DD_STEP_TWO, torch_cuda)
+    full = [r for r in tuner.sweep(bench, tuner.enumerate_space(), None, iterations=5) if r["valid"]]
+    best = min(r["ms"] for r in full)
+    pick = tuner.learned_tune(bench, model, 0.1, 5)
+    assert pick["timed"] == 72
+    assert pick["ms"] <= 1.05 * best, (pick, best)

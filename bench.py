@@ -14,7 +14,10 @@ replayed from a HIP graph.  The same work with one launch per beam and step
 Each persistent kernel's average duration is measured with HIP events on the
 stream it runs on and turned into an HBM-roofline fraction from its
 algorithmic bytes (input read once, intermediate written and read once,
-output written once, padded rows).
+output written once, padded rows).  The batch keeps its intermediate as
+uint16 sums (2 bytes per element instead of the reference's 4); the
+fraction is given for the executed data flow and, beside it, for the
+reference's float-intermediate data flow.
 """
 from __future__ import annotations
 
@@ -244,6 +247,7 @@ def run_batched(args, torch):
     dominant = max(avg, key=avg.get)
     achieved = step_bytes[dominant] / (avg[dominant] / 1e3)
     total_bytes = bs.bytes_moved()
+    reference_bytes = bs.bytes_moved(intermediate_bytes=4)  # float intermediate, the reference data flow
     value = units / (ms / 1e3)
     dms = sorted({w.obs.nr_dms_subbanding * w.obs.nr_dms for w in ws})
     workload = f"batched-beams-{len(ws)}x{ws[0].obs.nr_channels}ch-two-step"
@@ -271,7 +275,11 @@ def run_batched(args, torch):
         "plain_per_beam": {"value": units / (ms_plain / 1e3), "ms_per_step": ms_plain,
                            "launches_per_step": 2 * len(ws)},
         "batched_vs_plain": ms_plain / ms,
+        "intermediate": "uint16" if info["compact_intermediate"] else "float32",
+        "bytes_per_unit": total_bytes / units,
         "hbm_roofline_frac_path": total_bytes / (ms / 1e3) / HBM_PEAK,
+        "reference_flow_bytes_per_unit": reference_bytes / units,
+        "hbm_roofline_frac_reference_flow": reference_bytes / (ms / 1e3) / HBM_PEAK,
         "kernels_ms": {names[m]: v for m, v in avg.items()},
         "roofline": {"bound": "hbm", "kernel": names[dominant], "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,

@@ -18,6 +18,8 @@ from .workloads import PADDING, Workload
 class BeamSet:
     def __init__(self, torch, workloads: List[Workload], padding: int = PADDING, confs: Optional[dict] = None,
                  pulses: bool = True):
+        """confs: {mode: Conf} for the batch; None = the tuned batch
+        configurations packaged with the library (else the defaults)."""
         self.torch = torch
         self.workloads = workloads
         self.padding = padding
@@ -51,6 +53,10 @@ class BeamSet:
             self.mids.append(torch.empty(z.intermediate_elements, dtype=torch.float32, device="cuda"))
             self.outs.append(torch.empty(z.sub_output_elements, dtype=torch.float32, device="cuda"))
         self.batch = Batch()
+        if confs is None:
+            from .tuner import load_batch_confs
+
+            confs = load_batch_confs()
         if confs:
             for mode, conf in confs.items():
                 self.batch.set_conf(mode, conf)
@@ -67,15 +73,18 @@ class BeamSet:
     def run_mode(self, mode: int) -> None:
         self.batch.run_mode(mode, self.stream)
 
-    def step_bytes(self) -> dict:
+    def step_bytes(self, intermediate_bytes: Optional[int] = None) -> dict:
         """Compulsory HBM bytes of each step over the whole set (padded rows):
         step one reads the input and writes the intermediate, step two reads
-        the intermediate and writes the output."""
+        the intermediate and writes the output.  The intermediate element is
+        2 bytes when the batch keeps it compact (uint16), else 4."""
+        if intermediate_bytes is None:
+            intermediate_bytes = 2 if self.info()["compact_intermediate"] else 4
         one = two = 0
         for plan in self.plans:
             z = plan.sizes
-            one += z.sub_input_elements + 4 * z.intermediate_elements
-            two += 4 * z.intermediate_elements + 4 * z.sub_output_elements
+            one += z.sub_input_elements + intermediate_bytes * z.intermediate_elements
+            two += intermediate_bytes * z.intermediate_elements + 4 * z.sub_output_elements
         return {DD_STEP_ONE: one, DD_STEP_TWO: two}
 
     def run_per_job(self) -> None:
@@ -88,14 +97,11 @@ class BeamSet:
         """DM trials x samples x beams of the final outputs."""
         return self.info()["units"]
 
-    def bytes_moved(self) -> int:
+    def bytes_moved(self, intermediate_bytes: Optional[int] = None) -> int:
         """Compulsory HBM bytes of one pass: input read once, intermediate
         written and read once, output written once (padded rows)."""
-        total = 0
-        for plan in self.plans:
-            z = plan.sizes
-            total += z.sub_input_elements + 8 * z.intermediate_elements + 4 * z.sub_output_elements
-        return total
+        b = self.step_bytes(intermediate_bytes)
+        return b[DD_STEP_ONE] + b[DD_STEP_TWO]
 
     def close(self) -> None:
         self.batch.close()

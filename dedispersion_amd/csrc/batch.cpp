@@ -13,6 +13,13 @@
 // jobs, step one of the two-step jobs, step two of the two-step jobs.  The
 // kernel boundary between step one and step two orders every intermediate
 // write before its reads.
+//
+// Compact intermediate (the default when every two-step job allows it): a
+// batch owns its jobs' intermediate buffers, so step one writes its sums --
+// integers below 2^16 when 255 x the live channels of any subband fit --
+// as uint16 into the first half of each buffer, and step two reads them with
+// the uint16 kernel (kernels_u16_staged.hip).  Final outputs are
+// bit-identical; the intermediate buffers then hold uint16 sums.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -54,7 +61,12 @@ struct KindQueue {
 struct dd_batch {
   std::vector<Job> jobs;
   dd_conf conf[3]{};
+  bool compact_requested = true;
+  bool compact = false;  // in effect after build()
   bool built = false;
+  // one ordinary launch per job (float intermediate): dd_batch_run_per_job
+  std::vector<U8Launch> pj_single, pj_one;
+  std::vector<F32Launch> pj_two;
   KindQueue q[3];
   DeviceBuffer<uint32_t> heads;  // claim counters, kQueues * kHeadStride per kind
   uint64_t units = 0;
@@ -82,22 +94,16 @@ struct dd_batch {
 namespace {
 
 dd_conf batch_default_conf(dd_mode mode) {
-  // Staged pipeline shapes that fill the device for ARTS-sized beams.
+  // The fastest staged shape of an exhaustive sweep of both kernel kinds over
+  // 40 uneven 1536-channel beams on MI355X.
   dd_conf c{};
   c.threads_d0 = 64;
   c.split_batches = 0;
   c.local = 1;
-  if (mode == DD_STEP_TWO) {
-    c.threads_d1 = 4;
-    c.items_d0 = 8;
-    c.items_d1 = 4;
-    c.unroll = 8;
-  } else {
-    c.threads_d1 = 8;
-    c.items_d0 = 16;
-    c.items_d1 = 2;
-    c.unroll = 16;
-  }
+  c.threads_d1 = 4;
+  c.items_d0 = 16;
+  c.items_d1 = 8;
+  c.unroll = 8;
   return c;
 }
 
@@ -140,20 +146,31 @@ dd_status build(dd_batch *b) {
     DD_HIP(hipGetDevice(&dev));
     DD_HIP(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev));
   }
+  b->pj_single.clear();
+  b->pj_one.clear();
+  b->pj_two.clear();
+  // compact intermediate: every two-step job's subband sums fit 16 bits
+  b->compact = b->compact_requested;
+  for (const Job &job : b->jobs)
+    if (job.kind == kTwoStep && 255u * std::max(job.plan->max_live_sub, 1u) > 65535u) b->compact = false;
   for (size_t j = 0; j < b->jobs.size(); ++j) {
     const Job &job = b->jobs[j];
     const dd_observation &o = job.plan->obs;
     const dd_sizes &z = job.plan->sizes;
     dd_status s;
     if (job.kind == kSingleStep) {
-      U8Launch L;
+      U8Launch L, P;
       s = prepare_single(job.plan, b->conf[DD_SINGLE_STEP], job.in, job.out, job.first_sbeam, job.nr_sbeams, L);
       if (s == DD_OK) s = check_staged(L, DD_SINGLE_STEP, j);
+      if (s == DD_OK)
+        s = prepare_single(job.plan, job.plan->conf[DD_SINGLE_STEP], job.in, job.out, job.first_sbeam,
+                           job.nr_sbeams, P);
       if (s != DD_OK) return s;
       KindQueue &k = b->q[DD_SINGLE_STEP];
       merge_geometry(k, L, k.u8.empty());
       k.flush = k.flush || L.flush;
       k.u8.push_back(L);
+      b->pj_single.push_back(P);
       b->units += (uint64_t)L.grid.z * o.nr_dms * z.nr_samples;
     } else {
       U8Launch L1;
@@ -162,7 +179,25 @@ dd_status build(dd_batch *b) {
       if (s != DD_OK) return s;
       F32Launch L2;
       s = prepare_step_two(job.plan, b->conf[DD_STEP_TWO], job.mid, job.out, job.first_sbeam, job.nr_sbeams, L2);
-      if (s == DD_OK) s = check_staged(L2, DD_STEP_TWO, j);
+      if (s != DD_OK) return s;
+      // the per-job launches use each plan's own configuration
+      U8Launch P1;
+      F32Launch P2;
+      s = prepare_step_one(job.plan, job.plan->conf[DD_STEP_ONE], job.in, job.mid, P1);
+      if (s == DD_OK)
+        s = prepare_step_two(job.plan, job.plan->conf[DD_STEP_TWO], job.mid, job.out, job.first_sbeam,
+                             job.nr_sbeams, P2);
+      if (s != DD_OK) return s;
+      b->pj_one.push_back(P1);
+      b->pj_two.push_back(P2);
+      if (b->compact) {
+        L1.k.out_u16 = 1;  // vector stores of four uint16 sums need 8-byte aligned rows
+        L1.k.vec_store = (z.sub_out_pitch % 4 == 0) && (reinterpret_cast<uintptr_t>(job.mid) & 7) == 0;
+        s = prepare_step_two_u16(job.plan, b->conf[DD_STEP_TWO], job.mid, job.out, job.first_sbeam, job.nr_sbeams,
+                                 L2);
+      } else {
+        s = check_staged(L2, DD_STEP_TWO, j);
+      }
       if (s != DD_OK) return s;
       KindQueue &k1 = b->q[DD_STEP_ONE];
       merge_geometry(k1, L1, k1.u8.empty());
@@ -193,7 +228,8 @@ dd_status build(dd_batch *b) {
       std::vector<RowSumF32Params> params;
       for (auto &L : k.f32) params.push_back(L.k);
       DD_HIP(k.d_f32.upload(params));
-      DD_HIP(rowsum_f32_batch_occupancy(k.g, &per_cu));
+      if (b->compact) DD_HIP(rowsum_u16_batch_occupancy(k.g, &per_cu));
+      else DD_HIP(rowsum_f32_batch_occupancy(k.g, &per_cu));
     } else {
       std::vector<RowSumU8Params> params;
       for (auto &L : k.u8) params.push_back(L.k);
@@ -218,7 +254,8 @@ dd_status enqueue_kind(dd_batch *b, int m, hipStream_t s) {
   uint32_t *heads = b->heads.ptr + m * kQueues * kHeadStride;
   DD_HIP(launch_clear_words(heads, kQueues * kHeadStride, s));
   TileQueues tq{k.d_tiles.ptr, k.d_begin.ptr, heads};
-  if (m == DD_STEP_TWO) DD_HIP(launch_rowsum_f32_batch(k.d_f32.ptr, tq, k.g, k.workgroups, s));
+  if (m == DD_STEP_TWO && b->compact) DD_HIP(launch_rowsum_u16_batch(k.d_f32.ptr, tq, k.g, k.workgroups, s));
+  else if (m == DD_STEP_TWO) DD_HIP(launch_rowsum_f32_batch(k.d_f32.ptr, tq, k.g, k.workgroups, s));
   else DD_HIP(launch_rowsum_u8_batch(k.d_u8.ptr, tq, k.g, k.flush, k.workgroups, s));
   return DD_OK;
 }
@@ -323,14 +360,21 @@ dd_status dd_batch_run_per_job(dd_batch *batch, void *stream) {
   size_t i_single = 0, i_two = 0;
   for (const Job &job : batch->jobs) {
     if (job.kind == kSingleStep) {
-      s = run(batch->q[DD_SINGLE_STEP].u8[i_single++], (hipStream_t)stream);
+      s = run(batch->pj_single[i_single++], (hipStream_t)stream);
     } else {
-      s = run(batch->q[DD_STEP_ONE].u8[i_two], (hipStream_t)stream);
-      if (s == DD_OK) s = run(batch->q[DD_STEP_TWO].f32[i_two], (hipStream_t)stream);
+      s = run(batch->pj_one[i_two], (hipStream_t)stream);
+      if (s == DD_OK) s = run(batch->pj_two[i_two], (hipStream_t)stream);
       ++i_two;
     }
     if (s != DD_OK) return s;
   }
+  return DD_OK;
+}
+
+dd_status dd_batch_set_compact_intermediate(dd_batch *batch, int enable) {
+  if (!batch) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  batch->compact_requested = enable != 0;
+  batch->invalidate();
   return DD_OK;
 }
 
@@ -347,6 +391,7 @@ dd_status dd_batch_get_info(dd_batch *batch, dd_batch_info *info) {
     info->lds_bytes[m] = batch->q[m].n_tiles ? batch->q[m].g.lds_bytes : 0;
   }
   info->units = batch->units;
+  info->compact_intermediate = batch->compact ? 1 : 0;
   return DD_OK;
 }
 

@@ -44,6 +44,10 @@ struct RowSumU8Params {
   const float *table;         // float delay per row (samples per unit DM)
   float first_dm, dm_step;
   uint32_t n_tiles_x, n_tiles_y, n_tiles;  // 1-D grid decomposition
+  // Compact output (pipelined kernels): the sums are written as uint16 at
+  // the same element offsets (out is then a uint16 array); every sum must
+  // be below 2^16 (255 x live rows of any group <= 65535).
+  uint32_t out_u16;
 };
 
 // Shifted row sum over float rows: step two (include/Dedispersion.hpp:155-176).
@@ -74,6 +78,12 @@ struct RowSumF32Params {
   const float *table;         // float delay per subband
   float first_dm, dm_step;
   uint32_t n_tiles_x, n_tiles_y, n_tiles;
+  // uint16 rows (kernels_u16_staged.hip): `in` points to uint16 step-one sums
+  // with the pitches above counted in uint16 elements (in_floats = elements
+  // addressable) and win_floats the window in elements; combine_rows rows
+  // (1, 2 or 4) may be added as 16-bit pairs before widening, which needs
+  // combine_rows x 255 x (live channels of any subband) <= 65535.
+  uint32_t combine_rows;
 };
 
 // Geometry of one launch.
@@ -129,6 +139,14 @@ hipError_t launch_rowsum_f32_batch(const RowSumF32Params *d_jobs, const TileQueu
 hipError_t rowsum_u8_batch_occupancy(const Geometry &g, bool flush, int *per_cu);
 hipError_t rowsum_f32_batch_occupancy(const Geometry &g, int *per_cu);
 hipError_t launch_clear_words(uint32_t *d_words, uint32_t n, hipStream_t stream);
+// Step two over uint16 step-one sums (kernels_u16_staged.hip): the staged
+// pipeline with a window of g.win uint16 elements per row, two copies per
+// row (shifted by one element) so that every read is dword aligned.
+hipError_t launch_rowsum_u16_staged(const RowSumF32Params &p, const Geometry &g, hipStream_t stream);
+hipError_t launch_rowsum_u16_batch(const RowSumF32Params *d_jobs, const TileQueues &q, const Geometry &g,
+                                   uint32_t workgroups, hipStream_t stream);
+hipError_t rowsum_u16_batch_occupancy(const Geometry &g, int *per_cu);
+inline uint32_t u16_staged_chunks(uint32_t run) { return run / 8 + 1; }  // 16-byte loads per lane per row
 constexpr uint32_t kStagedSlotsPerWave = 2;  // rows staged per wavefront per group
 constexpr uint32_t kU8StagedChunks = 2;      // 16-byte staging loads per lane per row (window <= 2 KiB)
 inline uint32_t f32_staged_chunks(uint32_t run) { return run / 4 + 1; }

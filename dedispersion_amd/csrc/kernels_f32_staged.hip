@@ -107,11 +107,11 @@ __device__ __forceinline__ void f32_tile(const RowSumF32Params &p, uint32_t logi
 
   auto compute = [&](uint32_t buf, uint32_t n) {
     for (uint32_t g = 0; g < n; ++g) {
-      const uint32_t *mrow = meta + (buf * G + g) * TD + dw;
+      const uint32_t *mrow = meta + (buf * G + g) * TD + dw;  // broadcast reads
       const float *w = windows + (buf * G + g) * WF + lane;
 #pragma unroll
       for (int j = 0; j < DPW; ++j) {
-        const float *src = w + uniform(mrow[j]);
+        const float *src = w + mrow[j];
 #pragma unroll
         for (int k = 0; k < RUN; ++k) acc[j][k] += src[64 * k];
       }

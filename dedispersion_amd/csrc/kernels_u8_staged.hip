@@ -146,8 +146,31 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
     }
   };
 
+  // Rows are taken two at a time so that the two extracted pairs of a DM
+  // and sample group go into its accumulator with one three-input add.
   auto compute = [&](uint32_t buf, uint32_t n) {
-    for (uint32_t g = 0; g < n; ++g) {
+    uint32_t g = 0;
+    for (; g + 1 < n; g += 2) {
+      const uint32_t *m0 = meta + (buf * G + g) * TD + dw;
+      const uint32_t *m1 = m0 + TD;
+      const uint32_t *win0 = windows + (buf * G + g) * WD + lane;
+      const uint32_t *win1 = win0 + WD;
+#pragma unroll
+      for (int j = 0; j < DPW; ++j) {
+        const uint32_t o0 = uniform(m0[j]), o1 = uniform(m1[j]);
+        const uint32_t *src0 = win0 + (o0 >> 2), *src1 = win1 + (o1 >> 2);
+        const uint32_t se0 = sel_even_of(o0 & 3), sd0 = sel_odd_of(o0 & 3);
+        const uint32_t se1 = sel_even_of(o1 & 3), sd1 = sel_odd_of(o1 & 3);
+#pragma unroll
+        for (int k = 0; k < NG; ++k) {
+          const uint32_t lo0 = src0[64 * k], hi0 = src0[64 * k + 1];
+          const uint32_t lo1 = src1[64 * k], hi1 = src1[64 * k + 1];
+          even[j][k] += __builtin_amdgcn_perm(hi0, lo0, se0) + __builtin_amdgcn_perm(hi1, lo1, se1);
+          odd[j][k] += __builtin_amdgcn_perm(hi0, lo0, sd0) + __builtin_amdgcn_perm(hi1, lo1, sd1);
+        }
+      }
+    }
+    if (g < n) {
       const uint32_t *mrow = meta + (buf * G + g) * TD + dw;
       const uint32_t *win = windows + (buf * G + g) * WD + lane;
 #pragma unroll
@@ -192,6 +215,39 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
     cur ^= 1;
   }
   flush();
+
+  if (p.out_u16) {
+    // compact output: the four sums of a group as uint16 (each below 2^16)
+#pragma unroll
+    for (int j = 0; j < DPW; ++j) {
+      const uint32_t dm = d0 + dw + j;
+      if (dm >= p.n_dms) continue;
+      uint16_t *orow = reinterpret_cast<uint16_t *>(p.out) + (uint64_t)beam * p.out_beam_pitch +
+                       (uint64_t)group * p.out_group_pitch + (uint64_t)dm * p.out_dm_pitch;
+#pragma unroll
+      for (int k = 0; k < NG; ++k) {
+        uint32_t w0, w1;  // samples (0, 1) and (2, 3)
+        if constexpr (FLUSH) {
+          w0 = (wide[j][4 * k] & 0xFFFFu) | (wide[j][4 * k + 1] << 16);
+          w1 = (wide[j][4 * k + 2] & 0xFFFFu) | (wide[j][4 * k + 3] << 16);
+        } else {
+          w0 = __builtin_amdgcn_perm(odd[j][k], even[j][k], 0x05040100u);
+          w1 = __builtin_amdgcn_perm(odd[j][k], even[j][k], 0x07060302u);
+        }
+        const uint32_t t = t0 + 4 * lane + 256 * k;
+        if (p.vec_store && t + 3 < p.n_samples) {
+          u32x2 v = {w0, w1};
+          *reinterpret_cast<u32x2 *>(orow + t) = v;
+        } else {
+          const uint32_t e[4] = {w0 & 0xFFFFu, w0 >> 16, w1 & 0xFFFFu, w1 >> 16};
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (t + q < p.n_samples) orow[t + q] = (uint16_t)e[q];
+        }
+      }
+    }
+    return;
+  }
 
 #pragma unroll
   for (int j = 0; j < DPW; ++j) {

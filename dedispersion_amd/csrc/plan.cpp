@@ -111,6 +111,7 @@ dd_status rebuild_live(dd_plan *p) {
         if (p->h_zapped[ch] == 0) sub[(size_t)sb * n + counts[sb]++] = ch;
     DD_HIP(p->live_sub.upload(sub));
     DD_HIP(p->live_sub_count.upload(counts));
+    p->max_live_sub = *std::max_element(counts.begin(), counts.end());
   }
   return DD_OK;
 }
@@ -333,6 +334,29 @@ dd_status prepare_step_two(dd_plan *plan, const dd_conf &c, const float *d_inter
   k.n_tiles_x = L.grid.x;
   k.n_tiles_y = L.grid.y;
   k.n_tiles = L.grid.x * L.grid.y * L.grid.z;
+  return DD_OK;
+}
+
+dd_status prepare_step_two_u16(dd_plan *plan, const dd_conf &c, const void *d_intermediate, float *d_output,
+                               uint32_t first_sbeam, uint32_t nr_sbeams, F32Launch &L) {
+  dd_status s = prepare_step_two(plan, c, reinterpret_cast<const float *>(d_intermediate), d_output, first_sbeam,
+                                 nr_sbeams, L);
+  if (s != DD_OK) return s;
+  const uint32_t bound = 255u * std::max(plan->max_live_sub, 1u);
+  if (bound > 65535u) return fail(DD_ERR_CONFIGURATION, "subband sums do not fit 16 bits");
+  if (c.local != 1 || c.unroll > kStagedSlotsPerWave * c.threads_d1)
+    return fail(DD_ERR_CONFIGURATION, "the uint16 step two needs the staged pipeline (local = 1, unroll <= 2 threads_d1)");
+  const uint32_t ts = 64 * c.items_d0, tile = c.threads_d1 * c.items_d1;
+  const uint32_t win = roundup(ts + plan->ish_two.max_spread(tile) + 8, 8);
+  if (win / 8 > 64 * u16_staged_chunks(c.items_d0))
+    return fail(DD_ERR_CONFIGURATION, "step-two shift spread too large for the uint16 window");
+  L.g.win = win;
+  L.g.staged = 1;
+  L.g.lds_bytes = 4 * (roundup(2 * plan->obs.nr_subbands, 4) + 2 * c.unroll * win + 2 * c.unroll * tile);
+  if (L.g.lds_bytes > plan->max_lds)
+    return fail(DD_ERR_CONFIGURATION, "LDS demand " + std::to_string(L.g.lds_bytes) + " bytes exceeds the device limit");
+  L.k.win_floats = win;
+  L.k.combine_rows = 65535u / bound >= 4 ? 4 : 65535u / bound >= 2 ? 2 : 1;
   return DD_OK;
 }
 

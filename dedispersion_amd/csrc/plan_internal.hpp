@@ -88,6 +88,7 @@ struct dd_plan {
 
   std::vector<uint32_t> h_zapped;  // flags [pad(C)]
   ddamd::DeviceBuffer<uint32_t> live_single, live_single_count, live_sub, live_sub_count;
+  uint32_t max_live_sub = 0;  // most live channels in one subband
   ddamd::DeviceBuffer<uint32_t> map_single, map_two;
 };
 
@@ -127,6 +128,12 @@ dd_status prepare_step_one(dd_plan *plan, const dd_conf &c, const uint8_t *d_inp
                            U8Launch &L);
 dd_status prepare_step_two(dd_plan *plan, const dd_conf &c, const float *d_intermediate, float *d_output,
                            uint32_t first_sbeam, uint32_t nr_sbeams, F32Launch &L);
+// Step two over the uint16 sums that step one writes with out_u16 (a batch
+// that owns its intermediate): the launch of prepare_step_two with `in` read
+// as uint16 elements.  Fails with DD_ERR_CONFIGURATION when c does not select
+// the staged pipeline or a subband's sum could reach 2^16.
+dd_status prepare_step_two_u16(dd_plan *plan, const dd_conf &c, const void *d_intermediate, float *d_output,
+                               uint32_t first_sbeam, uint32_t nr_sbeams, F32Launch &L);
 // Launch a prepared launch (staged or direct kernel).
 dd_status run(const U8Launch &L, hipStream_t stream);
 dd_status run(const F32Launch &L, hipStream_t stream);

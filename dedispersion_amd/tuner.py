@@ -207,6 +207,77 @@ WORKLOADS = {
 }
 
 
+def staged_space(space: Dict = SPACE) -> List[Conf]:
+    """Configurations that select the staged pipeline (what a batch runs)."""
+    return [c for c in enumerate_space(space) if c.local == 1 and c.unroll <= 2 * c.threads_d1]
+
+
+def sweep_batch(beams, mode: int, confs: Iterable[Conf], iterations: int = 3, progress=None) -> List[Dict]:
+    """Times one kernel kind of a batch (beams.BeamSet) under each
+    configuration with HIP events around its persistent launch."""
+    import torch
+
+    out = []
+    for conf in confs:
+        try:
+            beams.set_conf(mode, conf)
+            beams.run_mode(mode)  # builds the queues, warms up
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(iterations):
+                beams.run_mode(mode)
+            b.record()
+            torch.cuda.synchronize()
+            rec = {"conf": asdict(conf), "valid": True, "ms": a.elapsed_time(b) / iterations}
+        except DDError as e:
+            rec = {"conf": asdict(conf), "valid": False, "error": str(e)}
+        out.append(rec)
+        if progress:
+            progress(rec)
+    return out
+
+
+BATCH_TUNED_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "batch_mi355x.json")
+
+
+def load_batch_confs(path: str = BATCH_TUNED_PATH) -> Dict[int, Conf]:
+    """Tuned batch configurations per kernel kind ({} when there are none)."""
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return {}
+    return {MODES[k]: Conf(**v["conf"]) for k, v in d.get("confs", {}).items()}
+
+
+def tune_batch(path: Optional[str], iterations: int = 3) -> Dict:
+    """Exhaustive sweep of the staged configurations of both kinds of the
+    40-beam batch; the best of each is written to `path`."""
+    import torch
+
+    from .beams import BeamSet
+
+    bs = BeamSet(torch, wl.batched_beams())
+    best = {}
+    try:
+        bs.run(use_graph=False)  # a valid intermediate for step two
+        for name in ("step_one", "step_two"):
+            t0 = time.time()
+            recs = sweep_batch(bs, MODES[name], staged_space(), iterations)
+            b = best_of(recs)
+            best[name] = {"conf": {k: v for k, v in b["conf"].items()}, "ms": b["ms"], "swept": len(recs)}
+            bs.set_conf(MODES[name], Conf(**b["conf"]))
+            print(f"batch {name}: {len(recs)} configurations in {time.time() - t0:.1f}s; best {b['conf']} "
+                  f"{b['ms']:.3f} ms", file=sys.stderr, flush=True)
+    finally:
+        bs.close()
+    out = {"workload": "batched-beams-40x1536ch-two-step", "device": "MI355X", "confs": best}
+    if path:
+        with open(path, "w") as f:
+            json.dump(out, f, indent=1)
+    return out
+
+
 def learned_tune(bench: Bench, model, budget: float, iterations: int) -> Dict:
     """Times the model's best-predicted fraction `budget` of the space."""
     from . import learned
@@ -220,7 +291,7 @@ def learned_tune(bench: Bench, model, budget: float, iterations: int) -> Dict:
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description="HIP kernel tuning: exhaustive sweep or learned pruning")
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="arts-two-step")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["batched-beams"], default="arts-two-step")
     ap.add_argument("--modes", default=None, help="comma list of single_step,step_one,step_two")
     ap.add_argument("--records", default=None, help="JSON-lines file to append to (resumable)")
     ap.add_argument("--tuned", default=None, help="write the best configurations in the reference format")
@@ -233,6 +304,9 @@ def main(argv=None):
     args = ap.parse_args(argv)
     import torch
 
+    if args.workload == "batched-beams":
+        tune_batch(args.tuned or BATCH_TUNED_PATH, args.iterations)
+        return
     make, modes = WORKLOADS[args.workload]
     if args.modes:
         modes = args.modes.split(",")

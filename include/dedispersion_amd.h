@@ -214,6 +214,7 @@ typedef struct dd_batch_info {
   uint32_t workgroups[3];   /* resident workgroups of each persistent launch */
   uint32_t lds_bytes[3];
   uint64_t units;           /* output elements of the final arrays (DM trials x samples x beams) */
+  uint32_t compact_intermediate; /* 1: step one writes uint16 sums (see dd_batch_set_compact_intermediate) */
 } dd_batch_info;
 
 dd_status dd_batch_create(dd_batch **batch);
@@ -227,6 +228,14 @@ dd_status dd_batch_add_single_step(dd_batch *batch, dd_plan *plan, const uint8_t
 dd_status dd_batch_add_two_step(dd_batch *batch, dd_plan *plan, const uint8_t *d_input, float *d_intermediate,
                                 float *d_output, uint32_t first_sbeam, uint32_t nr_sbeams);
 dd_status dd_batch_set_conf(dd_batch *batch, dd_mode mode, const dd_conf *conf);
+/* Compact intermediate (on by default): when 255 x the live channels of
+ * every subband of every two-step job is below 2^16, step one writes its
+ * sums as uint16 into the first half of each job's intermediate buffer
+ * (same element offsets) and step two adds them as integers.  The final
+ * outputs are bit-identical to the float pipeline; the intermediate
+ * buffers then hold uint16 sums.  With enable = 0, or when a job does not
+ * qualify, the batch keeps the float intermediate of the reference layout. */
+dd_status dd_batch_set_compact_intermediate(dd_batch *batch, int enable);
 dd_status dd_batch_get_conf(const dd_batch *batch, dd_mode mode, dd_conf *conf);
 /* Builds the work queues (first run, or after a change) and enqueues the
  * batch on `stream`.  use_graph != 0: the launches are captured once into a
@@ -237,7 +246,8 @@ dd_status dd_batch_run(dd_batch *batch, int use_graph, void *stream);
  * two reads what step one of the same batch wrote: run DD_STEP_ONE first. */
 dd_status dd_batch_run_mode(dd_batch *batch, dd_mode mode, void *stream);
 /* The same work as one ordinary launch per job and kernel kind, in job
- * order (the reference's launch pattern), with the batch configurations. */
+ * order (the reference's launch pattern), each with its plan's own
+ * configuration and the float intermediate of the reference layout. */
 dd_status dd_batch_run_per_job(dd_batch *batch, void *stream);
 dd_status dd_batch_get_info(dd_batch *batch, dd_batch_info *info);
 

@@ -72,13 +72,17 @@ class Jobs:
             o.fill_(-1.0)
         self.torch.cuda.synchronize()
 
-    def check(self, what):
+    def check(self, what, compact=False):
+        """compact: the intermediate buffers hold uint16 sums in their first half."""
         self.torch.cuda.synchronize()
         for j, (w, plan) in enumerate(zip(self.ws, self.plans)):
             z = plan.sizes
             if self.mids[j] is not None:
                 want = self.want_mid[j]
-                got = self.mids[j].cpu().numpy().reshape(want.shape)
+                raw = self.mids[j].cpu().numpy()
+                if compact:
+                    raw = raw.view(np.uint16)[: raw.size].astype(np.float32)
+                got = raw.reshape(want.shape)
                 compare_rows(got, want, z.nr_samples_subbanding, f"{what} job {j} step one")
                 want = self.want_out[j]
                 got = self.outs[j].cpu().numpy().reshape(want.shape)
@@ -89,23 +93,62 @@ class Jobs:
                 compare_rows(got, want, z.nr_samples, f"{what} job {j} single step")
 
 
-def test_batch_two_step_uneven_jobs(torch_cuda):
+@pytest.mark.parametrize("compact", [True, False], ids=["uint16-intermediate", "float-intermediate"])
+def test_batch_two_step_uneven_jobs(torch_cuda, compact):
     jobs = Jobs(torch_cuda, tiny_two_step_workloads())
     with Batch() as b:
         b.set_conf(DD_STEP_ONE, STAGED_SMALL)
         b.set_conf(DD_STEP_TWO, STAGED_STEP_TWO)
+        b.set_compact_intermediate(compact)
         for plan, d_in, d_mid, d_out in zip(jobs.plans, jobs.inputs, jobs.mids, jobs.outs):
             b.add_two_step(plan, d_in.data_ptr(), d_mid.data_ptr(), d_out.data_ptr())
         info = b.info()
         assert info["jobs"] == 4 and info["tiles"][0] == 0 and info["tiles"][1] > 0 and info["tiles"][2] > 0
+        assert info["compact_intermediate"] == int(compact)
         want_units = sum(w.obs.nr_synthesized_beams * w.obs.nr_dms_subbanding * w.obs.nr_dms *
                          w.obs.nr_samples_per_batch for w in jobs.ws)
         assert info["units"] == want_units
         b.run(use_graph=False, stream=stream_of(torch_cuda))
-        jobs.check("persistent")
+        jobs.check("persistent", compact=compact)
         jobs.reset()
         b.run_per_job(stream=stream_of(torch_cuda))
         jobs.check("per job")
+
+
+@pytest.mark.parametrize("conf", [Conf(64, t1, i0, i1, u, 1) for (t1, i0, i1, u) in
+                                  [(1, 4, 1, 1), (1, 16, 8, 2), (2, 8, 2, 3), (4, 4, 8, 8), (8, 16, 1, 16),
+                                   (8, 8, 4, 5), (4, 16, 2, 7)]],
+                         ids=lambda c: f"t{c.threads_d1}i{c.items_d0}x{c.items_d1}u{c.unroll}")
+def test_batch_uint16_step_two_configurations(torch_cuda, conf):
+    """The uint16 step two over every register shape and unroll, including
+    unrolls that do not divide the number of subbands."""
+    ws = tiny_two_step_workloads() + [workloads.tiny(channels=192, samples=900, dms=17, step=0.6, subbands=4,
+                                                      coarse=5, coarse_step=5.0, min_freq=1300.0, bandwidth=1.0,
+                                                      seed=35, zapped=(5, 77))]
+    jobs = Jobs(torch_cuda, ws)
+    with Batch() as b:
+        b.set_conf(DD_STEP_ONE, STAGED_SMALL)
+        b.set_conf(DD_STEP_TWO, conf)
+        for plan, d_in, d_mid, d_out in zip(jobs.plans, jobs.inputs, jobs.mids, jobs.outs):
+            b.add_two_step(plan, d_in.data_ptr(), d_mid.data_ptr(), d_out.data_ptr())
+        assert b.info()["compact_intermediate"] == 1
+        b.run(stream=stream_of(torch_cuda))
+        jobs.check(f"uint16 step two {conf}", compact=True)
+
+
+def test_batch_falls_back_to_float_for_wide_subbands(torch_cuda):
+    """300 channels per subband: 255 x 300 > 2^16, so the batch keeps the
+    float intermediate (and its results)."""
+    w = workloads.tiny(channels=600, samples=400, dms=6, step=0.4, subbands=2, coarse=3, coarse_step=2.0,
+                       min_freq=1300.0, bandwidth=1.0, seed=36)
+    jobs = Jobs(torch_cuda, [w])
+    with Batch() as b:
+        b.set_conf(DD_STEP_ONE, STAGED_SMALL)
+        b.set_conf(DD_STEP_TWO, STAGED_STEP_TWO)
+        b.add_two_step(jobs.plans[0], jobs.inputs[0].data_ptr(), jobs.mids[0].data_ptr(), jobs.outs[0].data_ptr())
+        assert b.info()["compact_intermediate"] == 0
+        b.run(stream=stream_of(torch_cuda))
+        jobs.check("wide subbands")
 
 
 def test_batch_graph_replays(torch_cuda):
@@ -123,7 +166,7 @@ def test_batch_graph_replays(torch_cuda):
             jobs.reset()
             b.run(use_graph=True, stream=stream_of(torch))
             _ = (scratch * 3.0).sum().item()  # eager work and a synchronization between replays
-            jobs.check(f"graph replay {replay}")
+            jobs.check(f"graph replay {replay}", compact=True)
 
 
 def test_batch_single_step_jobs(torch_cuda):
@@ -155,7 +198,7 @@ def test_batch_mixed_single_and_two_step(torch_cuda):
         b.add_single_step(one.plans[0], one.inputs[0].data_ptr(), one.outs[0].data_ptr())
         b.add_two_step(two.plans[1], two.inputs[1].data_ptr(), two.mids[1].data_ptr(), two.outs[1].data_ptr())
         b.run(stream=stream_of(torch_cuda))
-        two.check("mixed batch")
+        two.check("mixed batch", compact=True)
         one.check("mixed batch")
 
 
@@ -171,7 +214,7 @@ def test_batch_rejects_unstaged_configuration(torch_cuda):
             b.run(stream=stream_of(torch_cuda))
         b.set_conf(DD_STEP_ONE, STAGED_SMALL)
         b.run(stream=stream_of(torch_cuda))
-        jobs.check("after rejected configurations")
+        jobs.check("after rejected configurations", compact=True)
 
 
 def test_batched_beams_40_match_per_beam_launches(torch_cuda):
@@ -201,9 +244,12 @@ def test_batched_beams_40_match_per_beam_launches(torch_cuda):
             got = bs.outs[j].view(o.nr_dms_subbanding * o.nr_dms, z.out_pitch)[:, : z.nr_samples]
             want = ref_out[j].view(o.nr_dms_subbanding * o.nr_dms, z.out_pitch)[:, : z.nr_samples]
             assert torch.equal(got, want), f"beam {j} ({ws[j].name}): batch differs from per-beam launches"
-            gm = bs.mids[j].view(-1, z.sub_out_pitch)[:, : z.nr_samples_subbanding]
+            # the batch keeps its intermediate as uint16 sums in the buffer's first half
+            n = bs.mids[j].numel()
+            gm = bs.mids[j].view(torch.int16)[:n].view(-1, z.sub_out_pitch)[:, : z.nr_samples_subbanding]
+            gm = gm.to(torch.int32) & 0xFFFF
             wm = ref_mid[j].view(-1, z.sub_out_pitch)[:, : z.nr_samples_subbanding]
-            assert torch.equal(gm, wm), f"beam {j}: intermediate differs"
+            assert torch.equal(gm.to(torch.float32), wm), f"beam {j}: intermediate differs"
         del ref_mid, ref_out
         for j in (0, 7):
             plan = bs.plans[j]
@@ -212,7 +258,8 @@ def test_batched_beams_40_match_per_beam_launches(torch_cuda):
             orc = oracle.Oracle(o, bs.padding, threads=host_threads())
             flags = zapped_flags(plan, w.zapped)
             inp = bs.inputs[j].cpu().numpy()
-            mid = bs.mids[j].cpu().numpy()
+            raw = bs.mids[j].cpu().numpy()
+            mid = raw.view(np.uint16)[: raw.size].astype(np.float32)
             c = o.nr_dms_subbanding - 2
             want = orc.step_one(inp, flags, rows=(c, c + 2))
             got = mid.reshape(1, o.nr_dms_subbanding, o.nr_subbands, z.sub_out_pitch)
@@ -224,3 +271,35 @@ def test_batched_beams_40_match_per_beam_launches(torch_cuda):
     finally:
         bs.close()
         torch.cuda.empty_cache()
+
+
+def test_batch_uint16_sums_at_the_16_bit_bound(torch_cuda):
+    """All-255 input: every step-one sum is 255 x the subband's live channels
+    (12240 for 48 live channels), the largest uint16 sums the batch writes,
+    and step two adds 8 of them per output sample beyond 2^16; the output
+    must be 255 x all live channels."""
+    torch = torch_cuda
+    w = workloads.tiny(channels=384, samples=777, dms=9, step=0.7, subbands=8, coarse=4, coarse_step=3.0,
+                       min_freq=1300.0, bandwidth=1.0, zapped=(50, 51, 200), seed=37)
+    plan = Plan(w.obs, 128)
+    plan.set_zapped_channels(zapped_flags(plan, w.zapped))
+    z = plan.sizes
+    d_in = torch.full((z.sub_input_elements,), 255, dtype=torch.uint8, device="cuda")
+    d_mid = torch.full((z.intermediate_elements,), -1.0, dtype=torch.float32, device="cuda")
+    d_out = torch.full((z.sub_output_elements,), -1.0, dtype=torch.float32, device="cuda")
+    live = np.full(8, 48)
+    live[1] -= 2  # channels 50, 51
+    live[4] -= 1  # channel 200
+    for conf in (Conf(64, 4, 8, 4, 8, 1), Conf(64, 8, 16, 2, 16, 1), Conf(64, 1, 4, 1, 2, 1)):
+        with Batch() as b:
+            b.set_conf(DD_STEP_ONE, STAGED_SMALL)
+            b.set_conf(DD_STEP_TWO, conf)
+            b.add_two_step(plan, d_in.data_ptr(), d_mid.data_ptr(), d_out.data_ptr())
+            assert b.info()["compact_intermediate"] == 1
+            b.run(stream=stream_of(torch))
+            torch.cuda.synchronize()
+        raw = d_mid.cpu().numpy()
+        mid = raw.view(np.uint16)[: raw.size].reshape(w.obs.nr_dms_subbanding, 8, z.sub_out_pitch)
+        assert np.all(mid[:, :, : z.nr_samples_subbanding] == (255 * live)[None, :, None]), conf
+        out = d_out.cpu().numpy().reshape(-1, z.out_pitch)[:, : w.obs.nr_samples_per_batch]
+        assert np.all(out == 255.0 * live.sum()), conf

@@ -7,9 +7,11 @@ element of the final dedispersed output (beam x DM trial x sample).
 Default workload: 40 synthesized beams of 1536 channels with seeded uneven
 DM counts {512..4096} and sample counts {8192..32768}, two-step subband
 dedispersion (32 subbands, coarse DMs of 6.4, 32 fine DMs of 0.2), run as a
-batch: one persistent launch for step one of every beam and one for step two,
-replayed from a HIP graph.  The same work with one launch per beam and step
-(the reference's launch pattern) is timed beside it.
+batch replayed from a HIP graph: the beams are split into chunks, step one of
+a chunk and step two of the previous chunk run concurrently on two streams.
+The same work with one launch per beam and step, each beam with its plan's
+tuned configuration and the reference's float intermediate (the reference's
+launch pattern), is timed beside it.
 
 Each persistent kernel's average duration is measured with HIP events on the
 stream it runs on and turned into an HBM-roofline fraction from its
@@ -221,7 +223,8 @@ def run_batched(args, torch):
     wall = time.perf_counter() - t0
     ms = start.elapsed_time(stop) / args.steps
 
-    # per-kernel durations: the two persistent launches, events around each
+    # per-kernel durations: each kernel kind alone (its persistent launch over
+    # all beams), events around each
     names = {DD_STEP_ONE: "step_one_batch", DD_STEP_TWO: "step_two_batch"}
     pairs = {m: [] for m in names}
     for _ in range(args.steps):
@@ -269,7 +272,9 @@ def run_batched(args, torch):
                 "5% zapped channels per beam)",
         "config": {"workload": workload, "beams": len(ws),
                    "channels": ws[0].obs.nr_channels, "subbands": ws[0].obs.nr_subbands, "dms_per_beam": dms,
-                   "samples_per_beam": samples, "units_per_step": units, "padding": 128, "launch": "persistent+graph",
+                   "samples_per_beam": samples, "units_per_step": units, "padding": 128,
+                   "launch": (f"pipelined: {info['pipeline_chunks']} chunks on 2 streams, tile-list grids, HIP graph"
+                              if info["pipeline_chunks"] else "persistent work queues per kernel kind, HIP graph"),
                    "tiles": info["tiles"], "workgroups": info["workgroups"],
                    "conf": confs},
         "plain_per_beam": {"value": units / (ms_plain / 1e3), "ms_per_step": ms_plain,

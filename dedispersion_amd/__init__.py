@@ -262,6 +262,9 @@ class Batch:
         check(self.lib.dd_batch_get_conf(self.handle, mode, ctypes.byref(c)))
         return Conf.from_c(c)
 
+    def set_pipeline(self, chunks: int) -> None:
+        check(self.lib.dd_batch_set_pipeline(self.handle, chunks))
+
     def set_compact_intermediate(self, enable: bool) -> None:
         check(self.lib.dd_batch_set_compact_intermediate(self.handle, 1 if enable else 0))
 
@@ -278,7 +281,8 @@ class Batch:
         i = _lib.DDBatchInfo()
         check(self.lib.dd_batch_get_info(self.handle, ctypes.byref(i)))
         return {"jobs": i.jobs, "tiles": list(i.tiles), "workgroups": list(i.workgroups),
-                "lds_bytes": list(i.lds_bytes), "units": i.units, "compact_intermediate": i.compact_intermediate}
+                "lds_bytes": list(i.lds_bytes), "units": i.units, "compact_intermediate": i.compact_intermediate,
+                "pipeline_chunks": i.pipeline_chunks}
 
 
 def synth_gaussian_u8(d_data: int, rows: int, cols: int, pitch: int, mean: float, std: float, seed: int,

@@ -94,6 +94,7 @@ class DDBatchInfo(ctypes.Structure):
         ("lds_bytes", c_uint32 * 3),
         ("units", c_uint64),
         ("compact_intermediate", c_uint32),
+        ("pipeline_chunks", c_uint32),
     ]
 
 
@@ -155,6 +156,7 @@ PROTOTYPES = {
     "dd_batch_set_conf": (c_int, [c_void_p, c_int, POINTER(DDConf)]),
     "dd_batch_get_conf": (c_int, [c_void_p, c_int, POINTER(DDConf)]),
     "dd_batch_set_compact_intermediate": (c_int, [c_void_p, c_int]),
+    "dd_batch_set_pipeline": (c_int, [c_void_p, c_uint32]),
     "dd_batch_run": (c_int, [c_void_p, c_int, c_void_p]),
     "dd_batch_run_mode": (c_int, [c_void_p, c_int, c_void_p]),
     "dd_batch_run_per_job": (c_int, [c_void_p, c_void_p]),

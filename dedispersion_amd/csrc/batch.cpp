@@ -14,6 +14,15 @@
 // kernel boundary between step one and step two orders every intermediate
 // write before its reads.
 //
+// Pipelined mode (dd_batch_set_pipeline, chunks >= 2): the two-step jobs are
+// split, in order, into chunks of near-equal work; step one of chunk c runs
+// on one batch stream and step two of chunk c on another after an event, so
+// step two of chunk c overlaps step one of chunk c + 1 on the device.  Each
+// launch is a plain grid over the chunk's (job, tile) list, which lets the
+// dispatcher interleave the workgroups of both kernels on the compute units.
+// The fork and join are events, so the whole pipeline is captured in the
+// graph.
+//
 // Compact intermediate (the default when every two-step job allows it): a
 // batch owns its jobs' intermediate buffers, so step one writes its sums --
 // integers below 2^16 when 255 x the live channels of any subband fit --
@@ -74,6 +83,12 @@ struct dd_batch {
   hipStream_t capture = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // pipelined mode
+  uint32_t chunks = 3;                         // <= 1: one persistent launch per kind
+  std::vector<uint32_t> off1, n1, off2, n2;    // per chunk: tile list offsets and counts
+  DeviceBuffer<TileDesc> d_list1, d_list2;     // step one / step two tiles, chunk after chunk
+  hipStream_t side[2] = {nullptr, nullptr};    // step one, step two
+  std::vector<hipEvent_t> events;              // fork, join x2, one per chunk
 
   void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -88,6 +103,10 @@ struct dd_batch {
   ~dd_batch() {
     drop_graph();
     if (capture) (void)hipStreamDestroy(capture);
+    for (auto st : side)
+      if (st) (void)hipStreamDestroy(st);
+    for (auto e : events)
+      if (e) (void)hipEventDestroy(e);
   }
 };
 
@@ -243,6 +262,42 @@ dd_status build(dd_batch *b) {
     DD_HIP(k.d_begin.upload(queue_bounds(k.n_tiles)));
   }
   if (b->heads.count == 0) DD_HIP(b->heads.allocate(3 * kQueues * kHeadStride));
+  b->off1.clear();
+  b->n1.clear();
+  b->off2.clear();
+  b->n2.clear();
+  const size_t n_two = b->q[DD_STEP_ONE].u8.size();
+  if (b->chunks >= 2 && n_two > 0) {
+    // chunks of near-equal step-two tiles, jobs in order
+    uint64_t total = 0;
+    for (auto &L : b->q[DD_STEP_TWO].f32) total += L.k.n_tiles;
+    std::vector<TileDesc> t1, t2;
+    size_t j = 0;
+    uint64_t done = 0;
+    for (uint32_t c = 0; c < b->chunks && j < n_two; ++c) {
+      const uint64_t goal = total * (c + 1) / b->chunks;
+      b->off1.push_back((uint32_t)t1.size());
+      b->off2.push_back((uint32_t)t2.size());
+      do {  // at least one job per chunk; the last chunk takes the rest
+        for (uint32_t t = 0; t < b->q[DD_STEP_ONE].u8[j].k.n_tiles; ++t) t1.push_back(TileDesc{(uint32_t)j, t});
+        for (uint32_t t = 0; t < b->q[DD_STEP_TWO].f32[j].k.n_tiles; ++t) t2.push_back(TileDesc{(uint32_t)j, t});
+        done += b->q[DD_STEP_TWO].f32[j].k.n_tiles;
+        ++j;
+      } while (j < n_two && (done < goal || c + 1 == b->chunks));
+      b->n1.push_back((uint32_t)t1.size() - b->off1.back());
+      b->n2.push_back((uint32_t)t2.size() - b->off2.back());
+    }
+    DD_HIP(b->d_list1.upload(t1));
+    DD_HIP(b->d_list2.upload(t2));
+    for (auto &st : b->side)
+      if (!st) DD_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    const size_t need = 3 + b->n1.size();
+    while (b->events.size() < need) {
+      hipEvent_t e = nullptr;
+      DD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      b->events.push_back(e);
+    }
+  }
   b->built = true;
   return DD_OK;
 }
@@ -260,7 +315,36 @@ dd_status enqueue_kind(dd_batch *b, int m, hipStream_t s) {
   return DD_OK;
 }
 
+// Step one of chunk c on side[0], step two of chunk c on side[1] after it;
+// forked from and joined back into s with events.
+dd_status enqueue_pipelined(dd_batch *b, hipStream_t s) {
+  hipStream_t a = b->side[0], c2 = b->side[1];
+  hipEvent_t fork = b->events[0], join_a = b->events[1], join_b = b->events[2];
+  DD_HIP(hipEventRecord(fork, s));
+  DD_HIP(hipStreamWaitEvent(a, fork, 0));
+  DD_HIP(hipStreamWaitEvent(c2, fork, 0));
+  dd_status st = enqueue_kind(b, DD_SINGLE_STEP, a);
+  if (st != DD_OK) return st;
+  const KindQueue &k1 = b->q[DD_STEP_ONE], &k2 = b->q[DD_STEP_TWO];
+  for (size_t c = 0; c < b->n1.size(); ++c) {
+    if (b->n1[c])
+      DD_HIP(launch_rowsum_u8_list(k1.d_u8.ptr, b->d_list1.ptr + b->off1[c], b->n1[c], k1.g, k1.flush, a));
+    DD_HIP(hipEventRecord(b->events[3 + c], a));
+    DD_HIP(hipStreamWaitEvent(c2, b->events[3 + c], 0));
+    if (b->n2[c] && b->compact)
+      DD_HIP(launch_rowsum_u16_list(k2.d_f32.ptr, b->d_list2.ptr + b->off2[c], b->n2[c], k2.g, c2));
+    else if (b->n2[c])
+      DD_HIP(launch_rowsum_f32_list(k2.d_f32.ptr, b->d_list2.ptr + b->off2[c], b->n2[c], k2.g, c2));
+  }
+  DD_HIP(hipEventRecord(join_a, a));
+  DD_HIP(hipEventRecord(join_b, c2));
+  DD_HIP(hipStreamWaitEvent(s, join_a, 0));
+  DD_HIP(hipStreamWaitEvent(s, join_b, 0));
+  return DD_OK;
+}
+
 dd_status enqueue(dd_batch *b, hipStream_t s) {
+  if (!b->n1.empty()) return enqueue_pipelined(b, s);
   for (int m = 0; m < 3; ++m) {
     dd_status st = enqueue_kind(b, m, s);
     if (st != DD_OK) return st;
@@ -371,6 +455,14 @@ dd_status dd_batch_run_per_job(dd_batch *batch, void *stream) {
   return DD_OK;
 }
 
+dd_status dd_batch_set_pipeline(dd_batch *batch, uint32_t chunks) {
+  if (!batch) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  if (chunks > 64) return fail(DD_ERR_INVALID_ARGUMENT, "at most 64 pipeline chunks");
+  batch->chunks = chunks;
+  batch->invalidate();
+  return DD_OK;
+}
+
 dd_status dd_batch_set_compact_intermediate(dd_batch *batch, int enable) {
   if (!batch) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
   batch->compact_requested = enable != 0;
@@ -392,6 +484,7 @@ dd_status dd_batch_get_info(dd_batch *batch, dd_batch_info *info) {
   }
   info->units = batch->units;
   info->compact_intermediate = batch->compact ? 1 : 0;
+  info->pipeline_chunks = (uint32_t)batch->n1.size();
   return DD_OK;
 }
 

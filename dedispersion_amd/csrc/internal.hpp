@@ -137,6 +137,13 @@ hipError_t launch_rowsum_u8_batch(const RowSumU8Params *d_jobs, const TileQueues
 hipError_t launch_rowsum_f32_batch(const RowSumF32Params *d_jobs, const TileQueues &q, const Geometry &g,
                                    uint32_t workgroups, hipStream_t stream);
 hipError_t rowsum_u8_batch_occupancy(const Geometry &g, bool flush, int *per_cu);
+// One workgroup per listed descriptor (no work queue): grid = n.
+hipError_t launch_rowsum_u8_list(const RowSumU8Params *d_jobs, const TileDesc *d_tiles, uint32_t n, const Geometry &g,
+                                 bool flush, hipStream_t stream);
+hipError_t launch_rowsum_f32_list(const RowSumF32Params *d_jobs, const TileDesc *d_tiles, uint32_t n,
+                                  const Geometry &g, hipStream_t stream);
+hipError_t launch_rowsum_u16_list(const RowSumF32Params *d_jobs, const TileDesc *d_tiles, uint32_t n,
+                                  const Geometry &g, hipStream_t stream);
 hipError_t rowsum_f32_batch_occupancy(const Geometry &g, int *per_cu);
 hipError_t launch_clear_words(uint32_t *d_words, uint32_t n, hipStream_t stream);
 // Step two over uint16 step-one sums (kernels_u16_staged.hip): the staged

@@ -228,6 +228,15 @@ __global__ void __launch_bounds__(512) rowsum_u16_batch(const RowSumF32Params *_
   }
 }
 
+// One workgroup per listed (job, tile) descriptor, in XCD-aware order.
+template <int RUN, int DPW>
+__global__ void __launch_bounds__(512) rowsum_u16_list(const RowSumF32Params *__restrict__ jobs,
+                                                         const TileDesc *__restrict__ tiles, uint32_t n) {
+  extern __shared__ uint32_t lds[];
+  const TileDesc d = tiles[xcd_logical(blockIdx.x, n)];
+  u16_tile<RUN, DPW>(jobs[d.job], d.tile, lds);
+}
+
 template <typename F> hipError_t dispatch(const Geometry &g, F &&f) {
   auto by_dpw = [&](auto run) -> hipError_t {
     switch (g.items_d1) {
@@ -261,6 +270,15 @@ hipError_t launch_rowsum_u16_batch(const RowSumF32Params *d_jobs, const TileQueu
   return dispatch(g, [&](auto run, auto dpw) {
     hipLaunchKernelGGL((rowsum_u16_batch<run.value, dpw.value>), dim3(workgroups), dim3(64 * g.threads_d1),
                        g.lds_bytes, stream, d_jobs, q);
+    return hipGetLastError();
+  });
+}
+
+hipError_t launch_rowsum_u16_list(const RowSumF32Params *d_jobs, const TileDesc *d_tiles, uint32_t n,
+                                  const Geometry &g, hipStream_t stream) {
+  return dispatch(g, [&](auto run, auto dpw) {
+    hipLaunchKernelGGL((rowsum_u16_list<run.value, dpw.value>), dim3(n), dim3(64 * g.threads_d1), g.lds_bytes, stream,
+                       d_jobs, d_tiles, n);
     return hipGetLastError();
   });
 }

@@ -303,6 +303,15 @@ __global__ void __launch_bounds__(512) rowsum_u8_batch(const RowSumU8Params *__r
   }
 }
 
+// One workgroup per listed (job, tile) descriptor, in XCD-aware order.
+template <int RUN, int DPW, bool FLUSH>
+__global__ void __launch_bounds__(512) rowsum_u8_list(const RowSumU8Params *__restrict__ jobs,
+                                                        const TileDesc *__restrict__ tiles, uint32_t n) {
+  extern __shared__ uint32_t lds[];
+  const TileDesc d = tiles[xcd_logical(blockIdx.x, n)];
+  u8_tile<RUN, DPW, FLUSH>(jobs[d.job], d.tile, lds);
+}
+
 // Calls f(RUN, DPW, FLUSH) with the compile-time constants of geometry g.
 template <typename F> hipError_t dispatch(const Geometry &g, bool flush, F &&f) {
   auto by_dpw = [&](auto run, auto fl) -> hipError_t {
@@ -340,6 +349,15 @@ hipError_t launch_rowsum_u8_batch(const RowSumU8Params *d_jobs, const TileQueues
   return dispatch(g, flush, [&](auto run, auto dpw, auto fl) {
     hipLaunchKernelGGL((rowsum_u8_batch<run.value, dpw.value, fl.value>), dim3(workgroups), dim3(64 * g.threads_d1),
                        g.lds_bytes, stream, d_jobs, q);
+    return hipGetLastError();
+  });
+}
+
+hipError_t launch_rowsum_u8_list(const RowSumU8Params *d_jobs, const TileDesc *d_tiles, uint32_t n, const Geometry &g,
+                                 bool flush, hipStream_t stream) {
+  return dispatch(g, flush, [&](auto run, auto dpw, auto fl) {
+    hipLaunchKernelGGL((rowsum_u8_list<run.value, dpw.value, fl.value>), dim3(n), dim3(64 * g.threads_d1),
+                       g.lds_bytes, stream, d_jobs, d_tiles, n);
     return hipGetLastError();
   });
 }

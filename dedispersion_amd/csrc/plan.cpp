@@ -122,18 +122,24 @@ dd_conf default_conf(dd_mode mode) {
   dd_conf c{};
   c.threads_d0 = 64;
   c.split_batches = 0;
-  if (mode == DD_STEP_TWO) {
-    c.threads_d1 = 4;
-    c.items_d0 = 4;
-    c.items_d1 = 8;
-    c.unroll = 8;
-    c.local = 2;
-  } else {
+  c.local = 1;
+  c.unroll = 16;
+  switch (mode) {
+  case DD_SINGLE_STEP:
     c.threads_d1 = 8;
     c.items_d0 = 16;
     c.items_d1 = 2;
-    c.unroll = 16;
-    c.local = 1;
+    break;
+  case DD_STEP_ONE:
+    c.threads_d1 = 4;
+    c.items_d0 = 16;
+    c.items_d1 = 4;
+    break;
+  default:
+    c.threads_d1 = 8;
+    c.items_d0 = 8;
+    c.items_d1 = 4;
+    break;
   }
   return c;
 }

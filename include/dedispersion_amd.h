@@ -215,6 +215,7 @@ typedef struct dd_batch_info {
   uint32_t lds_bytes[3];
   uint64_t units;           /* output elements of the final arrays (DM trials x samples x beams) */
   uint32_t compact_intermediate; /* 1: step one writes uint16 sums (see dd_batch_set_compact_intermediate) */
+  uint32_t pipeline_chunks;      /* chunks of the pipelined mode in effect, 0 = off */
 } dd_batch_info;
 
 dd_status dd_batch_create(dd_batch **batch);
@@ -236,6 +237,14 @@ dd_status dd_batch_set_conf(dd_batch *batch, dd_mode mode, const dd_conf *conf);
  * buffers then hold uint16 sums.  With enable = 0, or when a job does not
  * qualify, the batch keeps the float intermediate of the reference layout. */
 dd_status dd_batch_set_compact_intermediate(dd_batch *batch, int enable);
+/* Pipelined mode (chunks >= 2, the default is 3; 0 or 1 = off): the two-step jobs are split,
+ * in order, into `chunks` groups of near-equal work.  Step one of group c
+ * and step two of group c run on two batch-owned streams, ordered by an
+ * event, so step two of group c overlaps step one of group c + 1; each
+ * launch is a plain grid over the group's tiles.  dd_batch_run forks from and
+ * joins back into the caller's stream (and its graph holds the whole DAG).
+ * dd_batch_run_mode keeps the persistent per-kind launches. */
+dd_status dd_batch_set_pipeline(dd_batch *batch, uint32_t chunks);
 dd_status dd_batch_get_conf(const dd_batch *batch, dd_mode mode, dd_conf *conf);
 /* Builds the work queues (first run, or after a change) and enqueues the
  * batch on `stream`.  use_graph != 0: the launches are captured once into a

@@ -93,18 +93,21 @@ class Jobs:
                 compare_rows(got, want, z.nr_samples, f"{what} job {j} single step")
 
 
+@pytest.mark.parametrize("chunks", [0, 2, 3, 9], ids=lambda c: f"pipeline{c}")
 @pytest.mark.parametrize("compact", [True, False], ids=["uint16-intermediate", "float-intermediate"])
-def test_batch_two_step_uneven_jobs(torch_cuda, compact):
+def test_batch_two_step_uneven_jobs(torch_cuda, compact, chunks):
     jobs = Jobs(torch_cuda, tiny_two_step_workloads())
     with Batch() as b:
         b.set_conf(DD_STEP_ONE, STAGED_SMALL)
         b.set_conf(DD_STEP_TWO, STAGED_STEP_TWO)
         b.set_compact_intermediate(compact)
+        b.set_pipeline(chunks)
         for plan, d_in, d_mid, d_out in zip(jobs.plans, jobs.inputs, jobs.mids, jobs.outs):
             b.add_two_step(plan, d_in.data_ptr(), d_mid.data_ptr(), d_out.data_ptr())
         info = b.info()
         assert info["jobs"] == 4 and info["tiles"][0] == 0 and info["tiles"][1] > 0 and info["tiles"][2] > 0
         assert info["compact_intermediate"] == int(compact)
+        assert (info["pipeline_chunks"] == 0) == (chunks < 2) and info["pipeline_chunks"] <= min(chunks, 4)
         want_units = sum(w.obs.nr_synthesized_beams * w.obs.nr_dms_subbanding * w.obs.nr_dms *
                          w.obs.nr_samples_per_batch for w in jobs.ws)
         assert info["units"] == want_units
@@ -151,14 +154,17 @@ def test_batch_falls_back_to_float_for_wide_subbands(torch_cuda):
         jobs.check("wide subbands")
 
 
-def test_batch_graph_replays(torch_cuda):
-    """The captured graph (queue reset kernel + persistent launches) gives the
-    eager result on every replay, with other device work between replays."""
+@pytest.mark.parametrize("chunks", [0, 2], ids=["persistent", "pipelined"])
+def test_batch_graph_replays(torch_cuda, chunks):
+    """The captured graph (queue reset kernel + persistent launches, or the
+    two-stream pipeline) gives the eager result on every replay, with other
+    device work between replays."""
     torch = torch_cuda
     jobs = Jobs(torch, tiny_two_step_workloads()[:3])
     with Batch() as b:
         b.set_conf(DD_STEP_ONE, STAGED_SMALL)
         b.set_conf(DD_STEP_TWO, STAGED_STEP_TWO)
+        b.set_pipeline(chunks)
         for plan, d_in, d_mid, d_out in zip(jobs.plans, jobs.inputs, jobs.mids, jobs.outs):
             b.add_two_step(plan, d_in.data_ptr(), d_mid.data_ptr(), d_out.data_ptr())
         scratch = torch.arange(1 << 20, device="cuda", dtype=torch.float32)
@@ -217,13 +223,16 @@ def test_batch_rejects_unstaged_configuration(torch_cuda):
         jobs.check("after rejected configurations", compact=True)
 
 
-def test_batched_beams_40_match_per_beam_launches(torch_cuda):
-    """The 40-beam uneven workload: the graph-replayed persistent batch equals
-    one launch per beam and step, bit for bit, on every beam; a few rows of
-    two beams are checked against the oracle directly."""
+@pytest.mark.parametrize("chunks", [0, 5], ids=["persistent", "pipelined"])
+def test_batched_beams_40_match_per_beam_launches(torch_cuda, chunks):
+    """The 40-beam uneven workload: the graph-replayed batch (persistent, or
+    pipelined over two streams) equals one launch per beam and step, bit for
+    bit, on every beam; a few rows of two beams are checked against the
+    oracle directly."""
     torch = torch_cuda
     ws = workloads.batched_beams()
     bs = BeamSet(torch, ws)
+    bs.batch.set_pipeline(chunks)
     try:
         info = bs.info()
         assert info["jobs"] == 40

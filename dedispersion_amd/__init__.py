@@ -104,14 +104,18 @@ def _f32_ptr(a: np.ndarray):
 class Plan:
     """A dd_plan: observation, padding, device-resident tables, launches."""
 
-    def __init__(self, obs: Observation, padding: int = 128):
+    def __init__(self, obs: Observation, padding: int = 128, input_bits: int = 8):
+        """input_bits: the reference's inputBits (1, 2, 4 = packed input rows, 8)."""
         self.lib = load()
         self.obs = obs
         self.padding = padding
+        self.input_bits = input_bits
         self._c_obs = obs.to_c()
         handle = ctypes.c_void_p()
         check(self.lib.dd_plan_create(ctypes.byref(handle), ctypes.byref(self._c_obs), padding))
         self.handle = handle
+        if input_bits != 8:
+            check(self.lib.dd_plan_set_input_bits(self.handle, input_bits))
         s = _lib.DDSizes()
         check(self.lib.dd_plan_get_sizes(self.handle, ctypes.byref(s)))
         self.sizes = s

@@ -83,6 +83,7 @@ class DDSizes(ctypes.Structure):
         ("max_shift", c_uint32),
         ("max_shift_step_one", c_uint32),
         ("max_shift_step_two", c_uint32),
+        ("input_bits", c_uint32),
     ]
 
 
@@ -134,6 +135,7 @@ PROTOTYPES = {
     "dd_plan_create": (c_int, [POINTER(c_void_p), POINTER(DDObservation), c_uint32]),
     "dd_plan_destroy": (c_int, [c_void_p]),
     "dd_plan_get_sizes": (c_int, [c_void_p, POINTER(DDSizes)]),
+    "dd_plan_set_input_bits": (c_int, [c_void_p, c_uint32]),
     "dd_plan_set_conf": (c_int, [c_void_p, c_int, POINTER(DDConf)]),
     "dd_plan_get_conf": (c_int, [c_void_p, c_int, POINTER(DDConf)]),
     "dd_plan_set_zapped_channels": (c_int, [c_void_p, POINTER(c_uint32)]),

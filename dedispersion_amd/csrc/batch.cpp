@@ -134,7 +134,7 @@ std::vector<uint32_t> queue_bounds(uint32_t n) {
 }
 
 template <typename L> dd_status check_staged(const L &launch, dd_mode mode, size_t job) {
-  if (!launch.g.staged)
+  if (launch.g.staged != 1)
     return fail(DD_ERR_CONFIGURATION, "batch configuration of mode " + std::to_string((int)mode) +
                                           " does not select the staged pipeline for job " + std::to_string(job) +
                                           " (needs local = 1, unroll <= 2 * threads_d1 and a window that fits)");
@@ -147,6 +147,7 @@ template <typename L> void merge_geometry(KindQueue &k, const L &launch, bool fi
   } else {
     k.g.lds_bytes = std::max(k.g.lds_bytes, launch.g.lds_bytes);
     k.g.win = std::max(k.g.win, launch.g.win);
+    k.g.packed |= launch.g.packed;
   }
 }
 

@@ -48,7 +48,34 @@ struct RowSumU8Params {
   // the same element offsets (out is then a uint16 array); every sum must
   // be below 2^16 (255 x live rows of any group <= 65535).
   uint32_t out_u16;
+  // Channel-block patterns (kernels_u8_factored.hip, single step):
+  // kFactoredRecordDwords-dword records per (DM tile of kFactoredDMs, block of
+  // four live channels), fact_slots Q rows per phase, fact_qw uint16 elements
+  // per Q row (a multiple of 16); win_bytes is then the staged row window.
+  const uint32_t *fact_records;
+  uint32_t fact_slots;
+  uint32_t fact_qw;
+  // Packed input (the reference's inputBits < 8, include/Dedispersion.hpp:92-103):
+  // log2(8 / inputBits) = 1, 2 or 3 for 4-, 2- and 1-bit samples, 0 for bytes.
+  // Sample x of a row is the field at bit (x % (8 / inputBits)) * inputBits
+  // of byte x / (8 / inputBits), least significant bit first; in_row_pitch and
+  // in_beam_pitch count bytes.  Only the pipelined kernels read packed rows
+  // (they unpack while staging into LDS).
+  uint32_t in_log2f;
 };
+
+// Records of the pattern kernel, one per (DM tile, block of four live channels):
+//   [0]      patterns P (<= kFactoredMaxPatterns) | rows in the block << 8
+//   [1]      shift of the block's last (base) channel at the tile's first DM
+//   [2], [3] per-row minimum / maximum of D_i over the patterns (bytes)
+//   [4..19]  patterns: D_i = shift(row i) - shift(base) as bytes
+//   [20..35] per DM of the tile, uint16: base-shift offset from the first DM
+//            (low byte) | pattern index << 8
+constexpr uint32_t kFactoredDMs = 32;
+constexpr uint32_t kFactoredMaxPatterns = 16;
+constexpr uint32_t kFactoredPatternWord = 4;
+constexpr uint32_t kFactoredDMWord = kFactoredPatternWord + kFactoredMaxPatterns;
+constexpr uint32_t kFactoredRecordDwords = kFactoredDMWord + kFactoredDMs / 2;
 
 // Shifted row sum over float rows: step two (include/Dedispersion.hpp:155-176).
 // Output element (sbeam, coarse, dm, t) = sum over subbands sb = 0..Nsb-1, in
@@ -95,6 +122,8 @@ struct Geometry {
   uint32_t win;        // LDS window per row (bytes for uint8 rows, floats for float rows)
   uint32_t lds_bytes;  // dynamic LDS per workgroup
   uint32_t staged;     // 1 = pipelined (double-buffered) kernel
+  uint32_t packed;     // 1 = the rows may be packed (RowSumU8Params::in_log2f)
+  uint32_t unroll;     // rows staged per barrier (the configuration's, or its cap)
 };
 
 // Launchers (kernels_u8.hip / kernels_f32.hip).  Return hipSuccess or the
@@ -105,6 +134,8 @@ hipError_t launch_rowsum_f32(const RowSumF32Params &p, const Geometry &g, dim3 g
                              hipStream_t stream);
 bool rowsum_u8_supported(uint32_t items_d0, uint32_t items_d1);
 bool rowsum_f32_supported(uint32_t items_d0, uint32_t items_d1);
+// Single step over channel-block patterns (kernels_u8_factored.hip); g.staged == 2.
+hipError_t launch_rowsum_u8_factored(const RowSumU8Params &p, const Geometry &g, hipStream_t stream);
 // Pipelined kernels (kernels_u8_staged.hip / kernels_f32_staged.hip).
 hipError_t launch_rowsum_u8_staged(const RowSumU8Params &p, const Geometry &g, bool flush, hipStream_t stream);
 hipError_t launch_rowsum_f32_staged(const RowSumF32Params &p, const Geometry &g, hipStream_t stream);

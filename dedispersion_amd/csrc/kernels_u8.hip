@@ -272,6 +272,7 @@ bool rowsum_u8_supported(uint32_t items_d0, uint32_t items_d1) {
 
 hipError_t launch_rowsum_u8(const RowSumU8Params &p, const Geometry &g, bool flush, dim3 grid,
                             hipStream_t stream) {
+  if (p.in_log2f != 0) return hipErrorInvalidValue;  // packed rows: pipelined kernels only
   return flush ? dispatch_run<true>(p, g, grid, stream) : dispatch_run<false>(p, g, grid, stream);
 }
 

@@ -16,6 +16,10 @@
 //    once at the start.
 //  * Workgroups are ordered XCD-aware (staged_common.hpp).
 //
+// Packed rows (inputBits 1, 2 or 4, include/Dedispersion.hpp:92-103) are
+// loaded as packed bytes and unpacked to one byte per sample as they are
+// written to LDS; the summation then reads the same windows as for bytes.
+//
 // The summation is that of kernels_u8.hip: lane l owns the four-sample groups
 // t0 + 4l + 256k, reads the two dwords of a group with one ds_read2_b32 and
 // adds v_perm_b32-extracted 16-bit pairs; partial sums are exact integers.
@@ -29,9 +33,64 @@ using namespace staged;
 __device__ __forceinline__ uint32_t sel_even_of(uint32_t r) { return r | 0x0c00u | ((r + 2) << 16) | 0x0c000000u; }
 __device__ __forceinline__ uint32_t sel_odd_of(uint32_t r) { return (r + 1) | 0x0c00u | ((r + 3) << 16) | 0x0c000000u; }
 
+// Unpacks 16 bytes of packed samples (log2 of samples per byte = lf, 1..3)
+// into one byte per sample at dst (16 << lf bytes).  Samples are the fields
+// of each byte from the least significant bit up, so dword e of the chunk
+// holds samples (32 >> (3 - lf)) * e onwards, in field order.
+__device__ __forceinline__ void unpack_store(uint32_t *dst, const u32x4 &v, uint32_t lf) {
+  if (lf == 1) {
+    // 4-bit fields: even fields to bytes of `lo`, odd fields to bytes of `hi`,
+    // then interleaved
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      u32x4 o;
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const uint32_t d = v[2 * h + e];
+        const uint32_t lo = d & 0x0F0F0F0Fu, hi = (d >> 4) & 0x0F0F0F0Fu;
+        o[2 * e] = __builtin_amdgcn_perm(hi, lo, 0x05010400u);
+        o[2 * e + 1] = __builtin_amdgcn_perm(hi, lo, 0x07030602u);
+      }
+      *reinterpret_cast<u32x4 *>(dst + 4 * h) = o;
+    }
+  } else if (lf == 2) {
+    // 2-bit fields: m[j] byte k = sample 4k + j; two rounds of byte interleave
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const uint32_t d = v[e];
+      const uint32_t m0 = d & 0x03030303u, m1 = (d >> 2) & 0x03030303u;
+      const uint32_t m2 = (d >> 4) & 0x03030303u, m3 = (d >> 6) & 0x03030303u;
+      const uint32_t a01 = __builtin_amdgcn_perm(m1, m0, 0x05010400u);  // s0 s1 s4 s5
+      const uint32_t a23 = __builtin_amdgcn_perm(m3, m2, 0x05010400u);  // s2 s3 s6 s7
+      const uint32_t b01 = __builtin_amdgcn_perm(m1, m0, 0x07030602u);  // s8 s9 s12 s13
+      const uint32_t b23 = __builtin_amdgcn_perm(m3, m2, 0x07030602u);  // s10 s11 s14 s15
+      u32x4 o;
+      o[0] = __builtin_amdgcn_perm(a23, a01, 0x05040100u);
+      o[1] = __builtin_amdgcn_perm(a23, a01, 0x07060302u);
+      o[2] = __builtin_amdgcn_perm(b23, b01, 0x05040100u);
+      o[3] = __builtin_amdgcn_perm(b23, b01, 0x07060302u);
+      *reinterpret_cast<u32x4 *>(dst + 4 * e) = o;
+    }
+  } else {
+    // 1-bit fields: a nibble n spread to bytes by n * (1 + 2^7 + 2^14 + 2^21),
+    // whose four shifted copies do not overlap, then bit 0 of each byte kept
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        u32x4 o;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          o[q] = (((v[e] >> (16 * h + 4 * q)) & 0xFu) * 0x204081u) & 0x01010101u;
+        *reinterpret_cast<u32x4 *>(dst + 8 * e + 4 * h) = o;
+      }
+    }
+  }
+}
+
 // One (beam x group, DM tile, sample tile) block: logical tile index
 // `logical` of the launch described by p (DM tile fastest).
-template <int RUN, int DPW, bool FLUSH>
+template <int RUN, int DPW, bool FLUSH, bool PACKED>
 __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logical, uint32_t *lds) {
   constexpr int NG = RUN / 4;   // four-sample groups per lane
   constexpr int TS = 64 * RUN;  // samples per tile
@@ -52,6 +111,8 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
   const uint32_t G = p.unroll;
   const uint32_t WD = p.win_bytes / 4;  // dwords per window
   const uint32_t NCH = WD / 4;          // 16-byte chunks per window
+  const uint32_t lf = PACKED ? p.in_log2f : 0u;  // log2 of samples per input byte
+  const uint32_t NCH_IN = NCH >> lf;    // 16-byte chunks of packed input per window
   const uint32_t dmax = p.n_dms - 1;
 
   // LDS: rows (packed row | beam << 16, float delay) | windows [2][G][WD] | shifts [2][G][TD]
@@ -81,16 +142,20 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
         const uint32_t row = packed & 0xFFFFu, in_beam = packed >> 16;
         const uint32_t sh = shift_of(p.first_dm, p.dm_step, min(d0 + lane, dmax), tab);
         const uint32_t shmin = uniform(sh);
-        const uint64_t off = uniform64((uint64_t)in_beam * p.in_beam_pitch + (uint64_t)row * p.in_row_pitch + t0 + shmin);
+        // first sample of the window and the byte that holds it (packed rows
+        // hold 1 << lf samples per byte)
+        const uint32_t x = t0 + shmin;
+        const uint64_t off = uniform64((uint64_t)in_beam * p.in_beam_pitch + (uint64_t)row * p.in_row_pitch + (x >> lf));
         const uint64_t a0 = off & ~uint64_t(3);
-        // byte offset of each DM's samples in the window (window starts at a0)
-        so[s] = sh - shmin + (uint32_t)(off & 3);
+        // offset, in unpacked samples, of each DM's samples in the window
+        // (the window starts at the first sample of byte a0)
+        so[s] = sh - shmin + ((uint32_t)(off & 3) << lf) + (x & ((1u << lf) - 1));
         const uint32_t recs = records_from(p.in_bytes, a0);
         const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(p.in + a0, recs);
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
           const uint32_t m = lane + 64 * c;
-          if (m < NCH) {
+          if (m < NCH_IN) {
             if (16 * m + 16 <= recs) {
               sv[s][c] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, 16 * m, 0, 0);
             } else {
@@ -108,10 +173,18 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
       const uint32_t g = wave + s * nwaves;
       if (g < nn) {
         uint32_t *w = windows + (buf * G + g) * WD;
+        if (!PACKED || lf == 0) {
 #pragma unroll
-        for (int c = 0; c < CPL; ++c) {
-          const uint32_t m = lane + 64 * c;
-          if (m < NCH) *reinterpret_cast<u32x4 *>(w + 4 * m) = sv[s][c];
+          for (int c = 0; c < CPL; ++c) {
+            const uint32_t m = lane + 64 * c;
+            if (m < NCH) *reinterpret_cast<u32x4 *>(w + 4 * m) = sv[s][c];
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) {
+            const uint32_t m = lane + 64 * c;
+            if (m < NCH_IN) unpack_store(w + ((4 * m) << lf), sv[s][c], lf);
+          }
         }
         if (lane < TD) meta[(buf * G + g) * TD + lane] = so[s];
       }
@@ -281,15 +354,15 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
   }
 }
 
-template <int RUN, int DPW, bool FLUSH>
+template <int RUN, int DPW, bool FLUSH, bool PACKED>
 __global__ void __launch_bounds__(512) rowsum_u8_staged(const RowSumU8Params p) {
   extern __shared__ uint32_t lds[];
-  u8_tile<RUN, DPW, FLUSH>(p, xcd_logical(blockIdx.x, p.n_tiles), lds);
+  u8_tile<RUN, DPW, FLUSH, PACKED>(p, xcd_logical(blockIdx.x, p.n_tiles), lds);
 }
 
 // Persistent form: workgroups take (beam, tile) descriptors from XCD-local
 // work queues until every queue is empty (see staged_common.hpp).
-template <int RUN, int DPW, bool FLUSH>
+template <int RUN, int DPW, bool FLUSH, bool PACKED>
 __global__ void __launch_bounds__(512) rowsum_u8_batch(const RowSumU8Params *__restrict__ beams,
                                                          const TileQueues q) {
   extern __shared__ uint32_t lds[];
@@ -299,46 +372,51 @@ __global__ void __launch_bounds__(512) rowsum_u8_batch(const RowSumU8Params *__r
     const uint32_t t = cur.next(q, &claimed);
     if (t == kNoTile) break;
     const TileDesc d = q.tiles[t];
-    u8_tile<RUN, DPW, FLUSH>(beams[d.job], d.tile, lds);
+    u8_tile<RUN, DPW, FLUSH, PACKED>(beams[d.job], d.tile, lds);
   }
 }
 
 // One workgroup per listed (job, tile) descriptor, in XCD-aware order.
-template <int RUN, int DPW, bool FLUSH>
+template <int RUN, int DPW, bool FLUSH, bool PACKED>
 __global__ void __launch_bounds__(512) rowsum_u8_list(const RowSumU8Params *__restrict__ jobs,
                                                         const TileDesc *__restrict__ tiles, uint32_t n) {
   extern __shared__ uint32_t lds[];
   const TileDesc d = tiles[xcd_logical(blockIdx.x, n)];
-  u8_tile<RUN, DPW, FLUSH>(jobs[d.job], d.tile, lds);
+  u8_tile<RUN, DPW, FLUSH, PACKED>(jobs[d.job], d.tile, lds);
 }
 
-// Calls f(RUN, DPW, FLUSH) with the compile-time constants of geometry g.
+// Calls f(RUN, DPW, FLUSH, PACKED) with the compile-time constants of
+// geometry g; PACKED kernels also read packed rows (g.packed: some job of the
+// launch has them), the others bytes only.
 template <typename F> hipError_t dispatch(const Geometry &g, bool flush, F &&f) {
-  auto by_dpw = [&](auto run, auto fl) -> hipError_t {
+  auto by_dpw = [&](auto run, auto fl, auto pk) -> hipError_t {
     switch (g.items_d1) {
-    case 1: return f(run, std::integral_constant<int, 1>{}, fl);
-    case 2: return f(run, std::integral_constant<int, 2>{}, fl);
-    case 4: return f(run, std::integral_constant<int, 4>{}, fl);
-    case 8: return f(run, std::integral_constant<int, 8>{}, fl);
+    case 1: return f(run, std::integral_constant<int, 1>{}, fl, pk);
+    case 2: return f(run, std::integral_constant<int, 2>{}, fl, pk);
+    case 4: return f(run, std::integral_constant<int, 4>{}, fl, pk);
+    case 8: return f(run, std::integral_constant<int, 8>{}, fl, pk);
     default: return hipErrorInvalidValue;
     }
   };
-  auto by_run = [&](auto fl) -> hipError_t {
+  auto by_run = [&](auto fl, auto pk) -> hipError_t {
     switch (g.items_d0) {
-    case 4: return by_dpw(std::integral_constant<int, 4>{}, fl);
-    case 8: return by_dpw(std::integral_constant<int, 8>{}, fl);
-    case 16: return by_dpw(std::integral_constant<int, 16>{}, fl);
+    case 4: return by_dpw(std::integral_constant<int, 4>{}, fl, pk);
+    case 8: return by_dpw(std::integral_constant<int, 8>{}, fl, pk);
+    case 16: return by_dpw(std::integral_constant<int, 16>{}, fl, pk);
     default: return hipErrorInvalidValue;
     }
   };
-  return flush ? by_run(std::true_type{}) : by_run(std::false_type{});
+  auto by_flush = [&](auto pk) -> hipError_t {
+    return flush ? by_run(std::true_type{}, pk) : by_run(std::false_type{}, pk);
+  };
+  return g.packed ? by_flush(std::true_type{}) : by_flush(std::false_type{});
 }
 
 } // namespace
 
 hipError_t launch_rowsum_u8_staged(const RowSumU8Params &p, const Geometry &g, bool flush, hipStream_t stream) {
-  return dispatch(g, flush, [&](auto run, auto dpw, auto fl) {
-    hipLaunchKernelGGL((rowsum_u8_staged<run.value, dpw.value, fl.value>), dim3(p.n_tiles), dim3(64 * g.threads_d1),
+  return dispatch(g, flush, [&](auto run, auto dpw, auto fl, auto pk) {
+    hipLaunchKernelGGL((rowsum_u8_staged<run.value, dpw.value, fl.value, pk.value>), dim3(p.n_tiles), dim3(64 * g.threads_d1),
                        g.lds_bytes, stream, p);
     return hipGetLastError();
   });
@@ -346,8 +424,8 @@ hipError_t launch_rowsum_u8_staged(const RowSumU8Params &p, const Geometry &g, b
 
 hipError_t launch_rowsum_u8_batch(const RowSumU8Params *d_jobs, const TileQueues &q, const Geometry &g, bool flush,
                                   uint32_t workgroups, hipStream_t stream) {
-  return dispatch(g, flush, [&](auto run, auto dpw, auto fl) {
-    hipLaunchKernelGGL((rowsum_u8_batch<run.value, dpw.value, fl.value>), dim3(workgroups), dim3(64 * g.threads_d1),
+  return dispatch(g, flush, [&](auto run, auto dpw, auto fl, auto pk) {
+    hipLaunchKernelGGL((rowsum_u8_batch<run.value, dpw.value, fl.value, pk.value>), dim3(workgroups), dim3(64 * g.threads_d1),
                        g.lds_bytes, stream, d_jobs, q);
     return hipGetLastError();
   });
@@ -355,16 +433,16 @@ hipError_t launch_rowsum_u8_batch(const RowSumU8Params *d_jobs, const TileQueues
 
 hipError_t launch_rowsum_u8_list(const RowSumU8Params *d_jobs, const TileDesc *d_tiles, uint32_t n, const Geometry &g,
                                  bool flush, hipStream_t stream) {
-  return dispatch(g, flush, [&](auto run, auto dpw, auto fl) {
-    hipLaunchKernelGGL((rowsum_u8_list<run.value, dpw.value, fl.value>), dim3(n), dim3(64 * g.threads_d1),
+  return dispatch(g, flush, [&](auto run, auto dpw, auto fl, auto pk) {
+    hipLaunchKernelGGL((rowsum_u8_list<run.value, dpw.value, fl.value, pk.value>), dim3(n), dim3(64 * g.threads_d1),
                        g.lds_bytes, stream, d_jobs, d_tiles, n);
     return hipGetLastError();
   });
 }
 
 hipError_t rowsum_u8_batch_occupancy(const Geometry &g, bool flush, int *per_cu) {
-  return dispatch(g, flush, [&](auto run, auto dpw, auto fl) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rowsum_u8_batch<run.value, dpw.value, fl.value>,
+  return dispatch(g, flush, [&](auto run, auto dpw, auto fl, auto pk) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, rowsum_u8_batch<run.value, dpw.value, fl.value, pk.value>,
                                                         (int)(64 * g.threads_d1), (size_t)g.lds_bytes);
   });
 }

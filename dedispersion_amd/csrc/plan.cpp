@@ -50,6 +50,28 @@ float subband_max_freq(const dd_observation &o) {
   return subband_min_freq(o) + span;
 }
 
+// log2 of the samples per input byte: 0 for bytes, 1, 2, 3 for 4-, 2- and
+// 1-bit samples.
+uint32_t input_log2f(const dd_plan *p) {
+  return p->input_bits == 1 ? 3u : p->input_bits == 2 ? 2u : p->input_bits == 4 ? 1u : 0u;
+}
+
+// Input row pitch and sizes.  A row is pad(S_disp / (8 / inputBits), padding)
+// bytes (include/Dedispersion.hpp:90, :97, :133, :140), the division rounded up
+// so that the row holds every sample when 8 / inputBits does not divide S_disp.
+void size_inputs(dd_plan *p) {
+  dd_sizes &z = p->sizes;
+  const uint32_t per = 1u << input_log2f(p);
+  const uint64_t C = p->obs.nr_channels;
+  z.input_bits = p->input_bits;
+  z.in_pitch = padto((z.nr_dispersed_samples + per - 1) / per, p->padding);
+  z.input_elements = (uint64_t)p->obs.nr_beams * C * z.in_pitch;
+  if (p->two_step) {
+    z.sub_in_pitch = padto((z.nr_dispersed_samples_subbanding + per - 1) / per, p->padding);
+    z.sub_input_elements = (uint64_t)p->obs.nr_beams * C * z.sub_in_pitch;
+  }
+}
+
 // src/DedispersionTest.cpp:129 / :145 / :146
 uint32_t dispersed(uint32_t samples, float shift0, float first, float step, uint32_t dms) {
   float last = first + ((float)(uint32_t)(dms - 1) * step);
@@ -88,6 +110,7 @@ dd_status rebuild_integer_tables(dd_plan *p) {
     s = build_integer(p, p->ish_two, p->shifts2.ptr, o.nr_subbands, o.nr_dms, o.first_dm, o.dm_step);
     if (s != DD_OK) return s;
   }
+  p->fact_valid = false;
   p->sizes.max_shift = p->ish_single.max_shift();
   p->sizes.max_shift_step_one = p->two_step ? p->ish_one.max_shift() : 0;
   p->sizes.max_shift_step_two = p->two_step ? p->ish_two.max_shift() : 0;
@@ -95,6 +118,7 @@ dd_status rebuild_integer_tables(dd_plan *p) {
 }
 
 dd_status rebuild_live(dd_plan *p) {
+  p->fact_valid = false;
   const uint32_t C = p->obs.nr_channels;
   std::vector<uint32_t> single, count1(1, 0);
   for (uint32_t ch = 0; ch < C; ++ch)
@@ -144,6 +168,81 @@ dd_conf default_conf(dd_mode mode) {
   return c;
 }
 
+// Pattern records of the single step for DM tiles of kFactoredDMs DMs and
+// blocks of four live channels (layout in internal.hpp).  fact_ok is false
+// when a record would need more than kFactoredMaxPatterns patterns or an
+// offset above 255.
+dd_status build_factored(dd_plan *p) {
+  if (p->fact_valid) return DD_OK;
+  p->fact_valid = true;
+  p->fact_ok = false;
+  const ShiftTable &t = p->ish_single;
+  std::vector<uint32_t> live;
+  for (uint32_t ch = 0; ch < p->obs.nr_channels; ++ch)
+    if (p->h_zapped[ch] == 0) live.push_back(ch);
+  if (live.empty()) return DD_OK;
+  const uint32_t D = p->obs.nr_dms, TD = kFactoredDMs, REC = kFactoredRecordDwords;
+  const uint32_t ny = (D + TD - 1) / TD, nb = ((uint32_t)live.size() + 3) / 4;
+  std::vector<uint32_t> rec((size_t)ny * nb * REC, 0);
+  uint32_t maxp = 0, base_spread = 0, delta_spread = 0;
+  for (uint32_t y = 0; y < ny; ++y) {
+    const uint32_t dfirst = y * TD;
+    for (uint32_t b = 0; b < nb; ++b) {
+      const uint32_t nrows = std::min<uint32_t>(4, (uint32_t)live.size() - 4 * b);
+      const uint32_t base = live[4 * b + nrows - 1];
+      uint32_t *r = rec.data() + ((size_t)y * nb + b) * REC;
+      const uint32_t s0 = t.at(base, dfirst);
+      std::vector<uint32_t> pats;
+      uint32_t mins = 0xFFFFFFFFu, maxs = 0;
+      for (uint32_t j = 0; j < TD; ++j) {
+        const uint32_t d = std::min(dfirst + j, D - 1);
+        const uint32_t sb = t.at(base, d);
+        if (sb < s0 || sb - s0 > 255) return DD_OK;
+        uint32_t pat = 0;
+        for (uint32_t i = 0; i < nrows; ++i) {
+          const uint32_t si = t.at(live[4 * b + i], d);
+          if (si < sb || si - sb > 255) return DD_OK;
+          pat |= (si - sb) << (8 * i);
+        }
+        uint32_t idx = 0;
+        while (idx < pats.size() && pats[idx] != pat) ++idx;
+        if (idx == pats.size()) {
+          if (pats.size() == kFactoredMaxPatterns) return DD_OK;
+          pats.push_back(pat);
+        }
+        r[kFactoredDMWord + j / 2] |= ((sb - s0) | (idx << 8)) << (16 * (j & 1));
+        base_spread = std::max(base_spread, sb - s0);
+      }
+      uint32_t lo = 0, hi = 0;
+      for (uint32_t i = 0; i < 4; ++i) {
+        uint32_t mn = 255, mx = 0;
+        for (uint32_t q : pats) {
+          const uint32_t v = i < nrows ? (q >> (8 * i)) & 0xFFu : 0;
+          mn = std::min(mn, v);
+          mx = std::max(mx, v);
+        }
+        lo |= mn << (8 * i);
+        hi |= mx << (8 * i);
+        delta_spread = std::max(delta_spread, mx - mn);
+      }
+      (void)mins;
+      (void)maxs;
+      r[0] = (uint32_t)pats.size() | (nrows << 8);
+      r[1] = s0;
+      r[2] = lo;
+      r[3] = hi;
+      for (size_t q = 0; q < pats.size(); ++q) r[kFactoredPatternWord + q] = pats[q];
+      maxp = std::max<uint32_t>(maxp, (uint32_t)pats.size());
+    }
+  }
+  DD_HIP(p->fact_records.upload(rec));
+  p->fact_max_patterns = maxp;
+  p->fact_base_spread = base_spread;
+  p->fact_delta_spread = delta_spread;
+  p->fact_ok = true;
+  return DD_OK;
+}
+
 } // namespace
 
 dd_status geometry_of(const dd_plan *p, dd_mode mode, const dd_conf &c, Geometry &g) {
@@ -156,7 +255,55 @@ dd_status geometry_of(const dd_plan *p, dd_mode mode, const dd_conf &c, Geometry
   g.threads_d1 = c.threads_d1;
   g.items_d0 = c.items_d0;
   g.items_d1 = c.items_d1;
-  if (c.local > 2) return fail(DD_ERR_CONFIGURATION, "local must be 0, 1 or 2");
+  g.packed = 0;
+  g.unroll = c.unroll;
+  if (mode != DD_STEP_TWO && p->input_bits < 8 &&
+      (c.local != 1 || c.unroll > kStagedSlotsPerWave * c.threads_d1)) {
+    // packed rows are unpacked as they are staged: only the pipelined kernel
+    // reads them, in the same shape
+    dd_conf staged = c;
+    staged.local = 1;
+    staged.unroll = std::min(c.unroll, kStagedSlotsPerWave * c.threads_d1);
+    return geometry_of(p, mode, staged, g);
+  }
+  if (c.local == 3) {
+    // channel-block patterns: single step only, 32-DM tiles
+    if (mode != DD_SINGLE_STEP) return fail(DD_ERR_CONFIGURATION, "local 3 (channel-block patterns) is single step only");
+    if ((c.items_d0 != 8 && c.items_d0 != 16) || (c.items_d1 != 4 && c.items_d1 != 8) ||
+        c.threads_d1 * c.items_d1 != kFactoredDMs)
+      return fail(DD_ERR_CONFIGURATION,
+                  "local 3 needs items_d0 8 or 16 and threads_d1 x items_d1 = 32 with items_d1 4 or 8");
+    if (c.unroll > 8) return fail(DD_ERR_CONFIGURATION, "local 3 takes at most 8 channel blocks per phase (unroll)");
+    dd_status s = build_factored(const_cast<dd_plan *>(p));
+    if (s != DD_OK) return s;
+    if (!p->fact_ok) {
+      // too many patterns or too large offsets: the staged pipeline, same shape
+      dd_conf staged = c;
+      staged.local = 1;
+      staged.unroll = std::min(c.unroll, kStagedSlotsPerWave * c.threads_d1);
+      return geometry_of(p, mode, staged, g);
+    }
+    const uint32_t ts = 64 * c.items_d0;
+    const uint32_t qw = roundup(ts + p->fact_base_spread + 2, 16);
+    const uint32_t rw = roundup(qw + p->fact_delta_spread + 3 + 8, 16);
+    const uint32_t slots = std::max<uint32_t>(p->fact_max_patterns, 4 * c.unroll);
+    g.threads_d1 = c.threads_d1;
+    g.items_d0 = c.items_d0;
+    g.items_d1 = c.items_d1;
+    g.local = 1;
+    g.staged = 2;
+    g.win = rw;
+    g.lds_bytes = 4 * (4 * c.unroll * (rw / 4) + slots * qw + c.unroll * kFactoredRecordDwords + 4 * c.unroll);
+    if (rw / 16 > 64 * 8 || g.lds_bytes > p->max_lds) {
+      // the phase does not fit in LDS: the staged pipeline, same shape
+      dd_conf staged = c;
+      staged.local = 1;
+      staged.unroll = std::min(c.unroll, kStagedSlotsPerWave * c.threads_d1);
+      return geometry_of(p, mode, staged, g);
+    }
+    return DD_OK;
+  }
+  if (c.local > 2) return fail(DD_ERR_CONFIGURATION, "local must be 0, 1, 2 or 3");
   g.local = c.local ? 1 : 0;
   g.win = 0;
   g.lds_bytes = 0;
@@ -179,14 +326,21 @@ dd_status geometry_of(const dd_plan *p, dd_mode mode, const dd_conf &c, Geometry
     if (!rowsum_u8_supported(c.items_d0, c.items_d1))
       return fail(DD_ERR_CONFIGURATION, "items_d0 must be 4, 8 or 16 and items_d1 1, 2, 4 or 8");
     if (g.local) {
-      // window start misalignment (< 4) + spread + the dword after each group
+      // window start misalignment (< 4 bytes, each of `per` samples) + spread
+      // + the dword after each group; whole 16-byte chunks of packed input
       const ShiftTable &t = mode == DD_SINGLE_STEP ? p->ish_single : p->ish_one;
-      g.win = roundup(ts + t.max_spread(tile) + 8, 16);
+      const uint32_t per = 1u << input_log2f(p);
+      g.win = roundup(ts + t.max_spread(tile) + 4 * per + 4, 16 * per);
       g.lds_bytes = c.unroll * g.win;
       if (stageable && g.win <= 16 * 64 * kU8StagedChunks) {
         const uint32_t live_pitch = mode == DD_SINGLE_STEP ? p->obs.nr_channels : cps(p);
         g.staged = 1;
+        g.packed = per > 1;
         g.lds_bytes = 4 * (roundup(2 * live_pitch, 4) + 2 * c.unroll * (g.win / 4) + 2 * c.unroll * tile);
+      } else if (per > 1) {
+        return fail(DD_ERR_CONFIGURATION, "packed input needs the pipelined kernel, whose row window is at most " +
+                                              std::to_string(16 * 64 * kU8StagedChunks) + " bytes; this one needs " +
+                                              std::to_string(g.win) + " (fewer samples per tile or DMs per tile)");
       }
     }
   }
@@ -207,7 +361,7 @@ dd_status prepare_single(dd_plan *plan, const dd_conf &c, const uint8_t *d_input
   if (nr_sbeams == 0) nr_sbeams = o.nr_synthesized_beams - std::min(first_sbeam, o.nr_synthesized_beams);
   if (nr_sbeams == 0 || first_sbeam + nr_sbeams > o.nr_synthesized_beams)
     return fail(DD_ERR_INVALID_ARGUMENT, "synthesized beam range out of bounds");
-  if ((uint64_t)z.nr_samples + z.max_shift > z.in_pitch)
+  if ((uint64_t)z.nr_samples + z.max_shift > (z.in_pitch << input_log2f(plan)))
     return fail(DD_ERR_INVALID_ARGUMENT, "dispersed batch shorter than samples + largest shift");
   dd_status s = geometry_of(plan, DD_SINGLE_STEP, c, L.g);
   if (s != DD_OK) return s;
@@ -226,13 +380,14 @@ dd_status prepare_single(dd_plan *plan, const dd_conf &c, const uint8_t *d_input
   k.first_sbeam = first_sbeam;
   k.in_row_pitch = z.in_pitch;
   k.in_beam_pitch = (uint64_t)o.nr_channels * z.in_pitch;
+  k.in_log2f = input_log2f(plan);
   k.out_dm_pitch = z.out_pitch;
   k.out_beam_pitch = (uint64_t)o.nr_dms * z.out_pitch;
   k.out_group_pitch = 0;
   k.n_samples = z.nr_samples;
   k.n_dms = o.nr_dms;
   k.n_groups = 1;
-  k.unroll = c.unroll;
+  k.unroll = L.g.unroll;
   k.win_bytes = L.g.win;
   k.nontemporal = 1;
   k.vec_store = (z.out_pitch % 4 == 0) && aligned16(d_output);
@@ -242,6 +397,12 @@ dd_status prepare_single(dd_plan *plan, const dd_conf &c, const uint8_t *d_input
   const uint32_t ts = 64 * c.items_d0, td = c.threads_d1 * c.items_d1;
   L.grid = dim3((z.nr_samples + ts - 1) / ts, (o.nr_dms + td - 1) / td, nr_sbeams);
   L.flush = o.nr_channels > 257;
+  if (L.g.staged == 2) {
+    k.fact_records = plan->fact_records.ptr;
+    k.fact_slots = std::max<uint32_t>(plan->fact_max_patterns, 4 * L.g.unroll);
+    k.fact_qw = roundup(ts + plan->fact_base_spread + 2, 16);
+    k.vec_store = (z.out_pitch % 2 == 0) && (reinterpret_cast<uintptr_t>(d_output) & 7) == 0;
+  }
   k.n_tiles_x = L.grid.x;
   k.n_tiles_y = L.grid.y;
   k.n_tiles = L.grid.x * L.grid.y * L.grid.z;
@@ -254,7 +415,7 @@ dd_status prepare_step_one(dd_plan *plan, const dd_conf &c, const uint8_t *d_inp
   if (!plan->two_step) return fail(DD_ERR_INVALID_ARGUMENT, "plan has no subband path");
   const dd_observation &o = plan->obs;
   const dd_sizes &z = plan->sizes;
-  if ((uint64_t)z.nr_samples_subbanding + z.max_shift_step_one > z.sub_in_pitch)
+  if ((uint64_t)z.nr_samples_subbanding + z.max_shift_step_one > (z.sub_in_pitch << input_log2f(plan)))
     return fail(DD_ERR_INVALID_ARGUMENT, "dispersed batch shorter than samples + largest shift");
   dd_status s = geometry_of(plan, DD_STEP_ONE, c, L.g);
   if (s != DD_OK) return s;
@@ -273,13 +434,14 @@ dd_status prepare_step_one(dd_plan *plan, const dd_conf &c, const uint8_t *d_inp
   k.first_sbeam = 0;
   k.in_row_pitch = z.sub_in_pitch;
   k.in_beam_pitch = (uint64_t)o.nr_channels * z.sub_in_pitch;
+  k.in_log2f = input_log2f(plan);
   k.out_dm_pitch = (uint64_t)o.nr_subbands * z.sub_out_pitch;
   k.out_beam_pitch = (uint64_t)o.nr_dms_subbanding * o.nr_subbands * z.sub_out_pitch;
   k.out_group_pitch = z.sub_out_pitch;
   k.n_samples = z.nr_samples_subbanding;
   k.n_dms = o.nr_dms_subbanding;
   k.n_groups = o.nr_subbands;
-  k.unroll = c.unroll;
+  k.unroll = L.g.unroll;
   k.win_bytes = L.g.win;
   k.nontemporal = 0;  // step two re-reads the intermediate
   k.vec_store = (z.sub_out_pitch % 4 == 0) && aligned16(d_intermediate);
@@ -328,7 +490,7 @@ dd_status prepare_step_two(dd_plan *plan, const dd_conf &c, const float *d_inter
   k.out_beam_pitch = (uint64_t)o.nr_dms_subbanding * o.nr_dms * z.out_pitch;
   k.n_samples = z.nr_samples;
   k.n_dms = o.nr_dms;
-  k.unroll = c.unroll;
+  k.unroll = L.g.unroll;
   k.win_floats = L.g.win;
   k.nontemporal = 1;
   k.vec_store = (z.out_pitch % 4 == 0) && aligned16(d_output);
@@ -367,6 +529,10 @@ dd_status prepare_step_two_u16(dd_plan *plan, const dd_conf &c, const void *d_in
 }
 
 dd_status run(const U8Launch &L, hipStream_t stream) {
+  if (L.g.staged == 2) {
+    DD_HIP(launch_rowsum_u8_factored(L.k, L.g, stream));
+    return DD_OK;
+  }
   if (L.g.staged) {
     DD_HIP(launch_rowsum_u8_staged(L.k, L.g, L.flush, stream));
     return DD_OK;
@@ -491,14 +657,14 @@ dd_status dd_plan_create(dd_plan **out, const dd_observation *obs_in, uint32_t p
                         o.nr_dms_subbanding);
   }
   const uint32_t C = o.nr_channels;
-  z.in_pitch = padto(z.nr_dispersed_samples, padding);
+  size_inputs(p);
   z.out_pitch = padto(z.nr_samples, padding / 4);
-  z.input_elements = (uint64_t)o.nr_beams * C * z.in_pitch;
   z.output_elements = (uint64_t)o.nr_synthesized_beams * o.nr_dms * z.out_pitch;
   if (p->two_step) {
-    z.sub_in_pitch = padto(z.nr_dispersed_samples_subbanding, padding);
+    // The intermediate rows are padded as for 8-bit input whatever inputBits
+    // is; include/Dedispersion.hpp:148 pads S1 / (8 / inputBits), which makes
+    // the rows of packed input overlap.
     z.sub_out_pitch = padto(z.nr_samples_subbanding, padding / 4);
-    z.sub_input_elements = (uint64_t)o.nr_beams * C * z.sub_in_pitch;
     z.intermediate_elements = (uint64_t)o.nr_beams * o.nr_dms_subbanding * o.nr_subbands * z.sub_out_pitch;
     z.sub_output_elements = (uint64_t)o.nr_synthesized_beams * o.nr_dms_subbanding * o.nr_dms * z.out_pitch;
   }
@@ -523,6 +689,15 @@ dd_status dd_plan_create(dd_plan **out, const dd_observation *obs_in, uint32_t p
   }
   for (int m = 0; m < 3; ++m) p->conf[m] = default_conf((dd_mode)m);
   *out = p;
+  return DD_OK;
+}
+
+dd_status dd_plan_set_input_bits(dd_plan *plan, uint32_t input_bits) {
+  if (!plan) return fail(DD_ERR_INVALID_ARGUMENT, "null plan");
+  if (input_bits != 1 && input_bits != 2 && input_bits != 4 && input_bits != 8)
+    return fail(DD_ERR_INVALID_ARGUMENT, "input_bits must be 1, 2, 4 or 8");
+  plan->input_bits = input_bits;
+  size_inputs(plan);
   return DD_OK;
 }
 

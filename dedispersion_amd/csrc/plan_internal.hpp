@@ -79,6 +79,7 @@ struct dd_plan {
   int device = 0;
   size_t max_lds = 65536;
   bool two_step = false;
+  uint32_t input_bits = 8;  // the reference's inputBits: 1, 2, 4 (packed rows) or 8
   dd_sizes sizes{};
   dd_conf conf[3]{};
 
@@ -89,6 +90,13 @@ struct dd_plan {
   std::vector<uint32_t> h_zapped;  // flags [pad(C)]
   ddamd::DeviceBuffer<uint32_t> live_single, live_single_count, live_sub, live_sub_count;
   uint32_t max_live_sub = 0;  // most live channels in one subband
+  // channel-block pattern records of the single step (built on first use)
+  ddamd::DeviceBuffer<uint32_t> fact_records;
+  bool fact_valid = false;
+  bool fact_ok = false;        // the pattern kernel can run this plan
+  uint32_t fact_max_patterns = 0;
+  uint32_t fact_base_spread = 0;   // largest base-shift offset inside a DM tile
+  uint32_t fact_delta_spread = 0;  // largest spread of one row's D_i over the patterns
   ddamd::DeviceBuffer<uint32_t> map_single, map_two;
 };
 

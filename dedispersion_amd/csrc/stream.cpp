@@ -155,6 +155,7 @@ dd_status dd_stream_create(dd_stream **out, dd_plan *plan, uint32_t nr_beams, ui
   if (!out || !plan) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
   *out = nullptr;
   if (!plan->two_step) return fail(DD_ERR_INVALID_ARGUMENT, "streaming runs the subband path: plan has no coarse DMs");
+  if (plan->input_bits != 8) return fail(DD_ERR_UNSUPPORTED, "streaming takes 8-bit input only");
   const dd_observation &o = plan->obs;
   if (o.nr_beams != 1 || o.nr_synthesized_beams != 1)
     return fail(DD_ERR_INVALID_ARGUMENT, "a streaming plan describes one beam (nr_beams = nr_synthesized_beams = 1)");

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define DD_ABI_VERSION 1
+#define DD_ABI_VERSION 2
 
 typedef enum dd_status {
   DD_OK = 0,
@@ -76,7 +76,14 @@ typedef struct dd_observation {
  *   local       0 = read rows through the caches; 1 = stage rows in LDS with
  *               a double-buffered pipeline (used when unroll <= 2 *
  *               threads_d1 and the row window fits, else as 2); 2 = stage
- *               rows in LDS, one group at a time
+ *               rows in LDS, one group at a time; 3 = single step over
+ *               channel-block patterns (blocks of four live channels whose
+ *               relative shifts repeat across a tile of 32 DMs are summed
+ *               once per pattern): needs threads_d1 x items_d1 = 32,
+ *               items_d1 4 or 8, items_d0 8 or 16, unroll (blocks per
+ *               phase) <= 8; plans whose shift table has more than 16
+ *               patterns per block and tile, or whose phase of `unroll`
+ *               blocks does not fit in LDS, run the staged pipeline
  *   split_batches must be 0 (unimplemented in the reference too). */
 typedef struct dd_conf {
   uint32_t threads_d0;
@@ -118,6 +125,7 @@ typedef struct dd_sizes {
   uint32_t max_shift;                  /* largest integer shift, single step */
   uint32_t max_shift_step_one;
   uint32_t max_shift_step_two;
+  uint32_t input_bits;                 /* bits per input sample (dd_plan_set_input_bits) */
 } dd_sizes;
 
 typedef struct dd_plan dd_plan;
@@ -136,6 +144,25 @@ uint32_t dd_abi_version(void);
 dd_status dd_plan_create(dd_plan **plan, const dd_observation *obs, uint32_t padding);
 dd_status dd_plan_destroy(dd_plan *plan);
 dd_status dd_plan_get_sizes(const dd_plan *plan, dd_sizes *sizes);
+
+/* Bits per input sample, the reference's `inputBits` argument
+ * (include/Dedispersion.hpp:59-60, :63-64): 8 (the default) or 1, 2, 4 for
+ * packed rows.  Packed rows hold 8 / input_bits samples per byte, sample x
+ * in the field at bit (x % (8 / input_bits)) * input_bits of byte
+ * x / (8 / input_bits), least significant bit first, read as an unsigned
+ * value (include/Dedispersion.hpp:94-101 and :137-144).  A row of the
+ * single-step and step-one inputs is then pad(ceil(S_disp / (8 / input_bits)),
+ * padding) bytes (the reference pads S_disp / (8 / input_bits) rounded
+ * down, :97, :140, which drops the last samples when 8 / input_bits does not
+ * divide S_disp); in_pitch, sub_in_pitch and the input sizes count bytes.
+ * Beams are input_bits-packed rows of C channels each, at the beam stride of
+ * the 8-bit layout (the reference's single-step sub-byte path, :97, omits
+ * the beam stride).  The intermediate keeps the 8-bit row padding (:148
+ * pads S1 / (8 / input_bits) there, which overlaps rows).  Packed rows are
+ * read only by the pipelined kernels, which unpack them while staging:
+ * configurations with local 0, 2 or 3 run as local 1 with unroll capped at
+ * 2 x threads_d1.  Streams (dd_stream_create) take 8-bit plans only. */
+dd_status dd_plan_set_input_bits(dd_plan *plan, uint32_t input_bits);
 
 /* Kernel configuration per mode (defaults are chosen at creation). */
 dd_status dd_plan_set_conf(dd_plan *plan, dd_mode mode, const dd_conf *conf);

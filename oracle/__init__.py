@@ -36,6 +36,7 @@ class OrcObs(ctypes.Structure):
         ("nr_dms_sub", c_uint32),
         ("first_dm_sub", c_float),
         ("dm_step_sub", c_float),
+        ("input_bits", c_uint32),
     ]
 
 
@@ -67,6 +68,8 @@ def load() -> ctypes.CDLL:
     lib.orc_get_shifts_step_two.argtypes = [P(OrcObs), c_uint32, P(c_float)]
     lib.orc_dispersed_samples.restype = c_uint32
     lib.orc_dispersed_samples.argtypes = [c_uint32, c_float, c_float, c_float, c_uint32]
+    lib.orc_input_pitch.restype = c_uint64
+    lib.orc_input_pitch.argtypes = [c_uint32, c_uint32, c_uint32]
     lib.orc_shift.restype = c_uint32
     lib.orc_shift.argtypes = [c_float, c_float, c_uint32, c_float]
     lib.orc_dedispersion_mt.restype = None
@@ -97,11 +100,14 @@ def pad(v: int, k: int) -> int:
 class Oracle:
     """The reference's CPU functions over one observation (numpy in/out)."""
 
-    def __init__(self, obs, padding: int, threads: int = 0):
+    def __init__(self, obs, padding: int, threads: int = 0, input_bits: int = 8):
         """obs: a dedispersion_amd.Observation-like object with the same field
-        names; dispersed sizes of 0 are derived with the reference's rules."""
+        names; dispersed sizes of 0 are derived with the reference's rules.
+        input_bits: the reference's inputBits (1, 2, 4 = packed rows, 8)."""
+        assert input_bits in (1, 2, 4, 8)
         self.lib = load()
         self.padding = padding
+        self.input_bits = input_bits
         self.threads = threads or max(1, os.cpu_count() or 1)
         o = OrcObs()
         o.nr_beams = obs.nr_beams
@@ -119,6 +125,7 @@ class Oracle:
         o.nr_dms_sub = obs.nr_dms_subbanding
         o.first_dm_sub = obs.first_dm_subbanding
         o.dm_step_sub = obs.dm_step_subbanding
+        o.input_bits = input_bits
         self.o = o
         self.shifts = self.get_shifts()
         self.shifts2 = self.get_shifts_step_two()
@@ -136,7 +143,8 @@ class Oracle:
     # sizes (elements)
     @property
     def in_pitch(self):
-        return pad(self.o.nr_samples_per_dispersed_batch, self.padding)
+        """Bytes per channel row of the single-step input."""
+        return self.lib.orc_input_pitch(self.o.nr_samples_per_dispersed_batch, self.input_bits, self.padding)
 
     @property
     def out_pitch(self):
@@ -144,7 +152,8 @@ class Oracle:
 
     @property
     def sub_in_pitch(self):
-        return pad(self.o.nr_samples_per_dispersed_batch_sub, self.padding)
+        """Bytes per channel row of the step-one input."""
+        return self.lib.orc_input_pitch(self.o.nr_samples_per_dispersed_batch_sub, self.input_bits, self.padding)
 
     @property
     def sub_out_pitch(self):

@@ -105,7 +105,34 @@ uint32_t orc_shift(float first_dm, float dm_step, uint32_t dm, float table) {
   return (uint32_t)delay;
 }
 
-/* include/Dedispersion.hpp:70-109, the inputBits >= 8 branch (:88-91). */
+static uint32_t orc_bits(const orc_obs *o) {
+  return (o->input_bits == 0 || o->input_bits >= 8) ? 8u : o->input_bits;
+}
+
+uint64_t orc_input_pitch(uint32_t dispersed_samples, uint32_t input_bits, uint32_t padding) {
+  if (input_bits == 0 || input_bits >= 8) return orc_pad(dispersed_samples, padding);
+  const uint32_t per_byte = 8 / input_bits;
+  return orc_pad((dispersed_samples + per_byte - 1) / per_byte, padding);
+}
+
+/* include/Dedispersion.hpp:94-101: the inputBits-bit field of sample x of a
+ * packed row, copied bit by bit, least significant bit first, into a value
+ * whose higher bits are zero.  Byte x / (8 / inputBits) holds the sample,
+ * starting at bit (x % (8 / inputBits)) * inputBits. */
+static uint32_t orc_packed_sample(const uint8_t *row, uint64_t x, uint32_t bits) {
+  const uint32_t per_byte = 8 / bits;
+  const uint8_t buffer = row[x / per_byte];
+  const uint32_t first_bit = (uint32_t)(x % per_byte) * bits;
+  uint32_t value = 0;
+  for (uint32_t bit = 0; bit < bits; bit++) value |= ((uint32_t)(buffer >> (first_bit + bit)) & 1u) << bit;
+  return value;
+}
+
+/* include/Dedispersion.hpp:70-109: the inputBits >= 8 branch (:88-91) and the
+ * sub-byte branch (:92-103).  The sub-byte branch of the reference indexes
+ * the input with the mapped beam number alone (:97, no beam stride), which
+ * only reads the right row for beam 0; here both branches use the beam
+ * stride of :90. */
 void orc_dedispersion(const orc_obs *o, const uint32_t *zapped,
                       const uint32_t *beam_mapping, const uint8_t *input,
                       float *output, const float *shifts, uint32_t padding,
@@ -113,7 +140,8 @@ void orc_dedispersion(const orc_obs *o, const uint32_t *zapped,
   const uint32_t C = o->nr_channels;
   const uint32_t D = o->nr_dms;
   const uint32_t samples = o->nr_samples_per_batch / o->downsampling;      /* :76 */
-  const uint64_t in_pitch = orc_pad(o->nr_samples_per_dispersed_batch, padding / sizeof(uint8_t));
+  const uint32_t bits = orc_bits(o);
+  const uint64_t in_pitch = orc_input_pitch(o->nr_samples_per_dispersed_batch, bits, padding / sizeof(uint8_t));
   const uint64_t out_pitch = orc_pad(samples, padding / sizeof(float));
   const uint64_t map_pitch = orc_pad(C, padding / sizeof(uint32_t));
   for (uint64_t row = row_begin; row < row_end; row++) {
@@ -125,15 +153,21 @@ void orc_dedispersion(const orc_obs *o, const uint32_t *zapped,
         uint32_t shift = orc_shift(o->first_dm, o->dm_step, dm, shifts[channel]); /* :81 */
         if (zapped[channel] != 0) continue;                              /* :83 */
         uint64_t beam = beam_mapping[sbeam * map_pitch + channel];
-        uint64_t idx = beam * C * in_pitch + (uint64_t)channel * in_pitch + sample + shift; /* :90 */
-        acc += (float)input[idx];
+        const uint8_t *row = input + beam * C * in_pitch + (uint64_t)channel * in_pitch; /* :90 */
+        if (bits == 8)
+          acc += (float)row[sample + shift];
+        else
+          acc += (float)orc_packed_sample(row, (uint64_t)sample + shift, bits); /* :94-101 */
       }
       output[sbeam * D * out_pitch + (uint64_t)dm * out_pitch + sample] = acc; /* :105 */
     }
   }
 }
 
-/* include/Dedispersion.hpp:111-153, the inputBits >= 8 branch (:131-134). */
+/* include/Dedispersion.hpp:111-153: the inputBits >= 8 branch (:131-134)
+ * and the sub-byte branch (:135-146).  The output rows are padded as the
+ * 8-bit case pads them; the reference pads S1 / (8 / inputBits) there (:148),
+ * which makes consecutive sub-byte rows overlap. */
 void orc_step_one(const orc_obs *o, const uint32_t *zapped,
                   const uint8_t *input, float *output, const float *shifts,
                   uint32_t padding, uint64_t row_begin, uint64_t row_end) {
@@ -142,8 +176,8 @@ void orc_step_one(const orc_obs *o, const uint32_t *zapped,
   const uint32_t cps = orc_channels_per_subband(o);
   const uint32_t Dc = o->nr_dms_sub;
   const uint32_t samples = o->nr_samples_per_batch_sub / o->downsampling; /* :119 */
-  const uint64_t in_pitch = orc_pad(o->nr_samples_per_dispersed_batch_sub, padding / sizeof(uint8_t));
-  /* :148 pads S1 / (8 / inputBits) = S1 for 8-bit input */
+  const uint32_t bits = orc_bits(o);
+  const uint64_t in_pitch = orc_input_pitch(o->nr_samples_per_dispersed_batch_sub, bits, padding / sizeof(uint8_t));
   const uint64_t out_pitch = orc_pad(o->nr_samples_per_batch_sub, padding / sizeof(float));
   for (uint64_t row = row_begin; row < row_end; row++) {
     const uint64_t beam = row / Dc;                                      /* :113 */
@@ -156,8 +190,11 @@ void orc_step_one(const orc_obs *o, const uint32_t *zapped,
           float relative = shifts[channel] - top;                        /* :124 */
           uint32_t shift = orc_shift(o->first_dm_sub, o->dm_step_sub, dm, relative);
           if (zapped[channel] != 0) continue;                            /* :126 */
-          uint64_t idx = beam * C * in_pitch + (uint64_t)channel * in_pitch + sample + shift; /* :133 */
-          acc += (float)input[idx];
+          const uint8_t *row = input + beam * C * in_pitch + (uint64_t)channel * in_pitch; /* :133, :140 */
+          if (bits == 8)
+            acc += (float)row[sample + shift];
+          else
+            acc += (float)orc_packed_sample(row, (uint64_t)sample + shift, bits); /* :137-144 */
         }
         output[beam * Dc * Nsb * out_pitch + (uint64_t)dm * Nsb * out_pitch +
                (uint64_t)subband * out_pitch + sample] = acc;           /* :148 */

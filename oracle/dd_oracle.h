@@ -62,6 +62,9 @@ typedef struct orc_obs {
   uint32_t nr_dms_sub;                          /* Dc      */
   float first_dm_sub;
   float dm_step_sub;
+  /* The reference's inputBits argument (include/Dedispersion.hpp:59-60):
+   * 1, 2 or 4 = packed samples, 8 (or 0) = one byte per sample. */
+  uint32_t input_bits;
 } orc_obs;
 
 /* isa::utils::pad: round v up to a multiple of k; k == 0 means no padding. */
@@ -86,7 +89,12 @@ uint32_t orc_dispersed_samples(uint32_t samples, float shift0, float first_dm,
 /* The per-(channel, DM) integer shift of include/Dedispersion.hpp:81,124,168. */
 uint32_t orc_shift(float first_dm, float dm_step, uint32_t dm, float table);
 
-/* include/Dedispersion.hpp:70-109 (inputBits = 8).
+/* Bytes per channel row of the input: pad(S_disp / (8 / inputBits), padding)
+ * as include/Dedispersion.hpp:90,97,133,140 pad it, with the division
+ * rounded up so that a row holds all S_disp samples. */
+uint64_t orc_input_pitch(uint32_t dispersed_samples, uint32_t input_bits, uint32_t padding);
+
+/* include/Dedispersion.hpp:70-109.
  * Rows [row_begin, row_end) of the flattened (sBeam, dm) output are
  * computed; pass 0 and sBeams*D for the whole output. */
 void orc_dedispersion(const orc_obs *o, const uint32_t *zapped,
@@ -94,7 +102,7 @@ void orc_dedispersion(const orc_obs *o, const uint32_t *zapped,
                       float *output, const float *shifts, uint32_t padding,
                       uint64_t row_begin, uint64_t row_end);
 
-/* include/Dedispersion.hpp:111-153 (inputBits = 8).  Rows are the flattened
+/* include/Dedispersion.hpp:111-153.  Rows are the flattened
  * (beam, coarse dm) pairs. */
 void orc_step_one(const orc_obs *o, const uint32_t *zapped,
                   const uint8_t *input, float *output, const float *shifts,

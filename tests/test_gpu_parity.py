@@ -64,6 +64,9 @@ SWEEP_CONFS = [
     for (t1, i0, i1, u, loc) in itertools.product((1, 2, 4, 8), (4, 8, 16), (1, 2, 4, 8), (1, 3, 8), (1, 2, 0))
     if t1 * i1 <= 64
 ]
+# channel-block patterns (local 3): 32-DM tiles
+PATTERN_CONFS = [Conf(64, t1, i0, i1, u, 3) for (t1, i1) in ((8, 4), (4, 8)) for i0 in (8, 16) for u in (1, 3, 8)]
+SWEEP_CONFS += PATTERN_CONFS
 
 
 @pytest.fixture(scope="module")
@@ -99,6 +102,34 @@ def test_single_step_conf_sweep(torch_cuda, ragged_case, conf):
     assert np.all(got[..., w.obs.nr_samples_per_batch:] == -1.0)
 
 
+@pytest.mark.parametrize("conf", PATTERN_CONFS, ids=lambda c: f"t{c.threads_d1}i{c.items_d0}x{c.items_d1}u{c.unroll}")
+def test_single_step_patterns_partial_block(torch_cuda, conf):
+    """Channel-block patterns with 253 live channels (a last block of one
+    channel), 70 DMs (a partial 32-DM tile) and 2100 samples (a partial
+    sample tile), padding 128."""
+    w = workloads.tiny(channels=257, samples=2100, dms=70, step=0.2, min_freq=1300.0, bandwidth=0.5,
+                       zapped=(3, 4, 5, 6, 7, 100), seed=17)
+    plan, inp, out, orc, flags = single_step(torch_cuda, w, padding=128, conf=conf, seed=17)
+    want = orc.dedispersion(inp, flags)
+    compare_rows(out.reshape(want.shape), want, w.obs.nr_samples_per_batch, f"patterns {conf}")
+
+
+def test_single_step_patterns_fall_back_when_too_many(torch_cuda):
+    """A coarse DM grid over narrow channels gives more than 16 patterns per
+    block and tile; local 3 then runs the staged pipeline, same results."""
+    w = workloads.tiny(channels=64, samples=600, dms=40, step=4.0, min_freq=1300.0, bandwidth=1.0, seed=18)
+    plan, inp, out, orc, flags = single_step(torch_cuda, w, padding=128, conf=Conf(64, 8, 8, 4, 4, 3), seed=18)
+    want = orc.dedispersion(inp, flags)
+    compare_rows(out.reshape(want.shape), want, w.obs.nr_samples_per_batch, "fallback")
+
+
+def test_single_step_patterns_plain_128(torch_cuda):
+    w = workloads.plain_128()
+    plan, inp, out, orc, flags = single_step(torch_cuda, w, conf=Conf(64, 8, 16, 4, 8, 3))
+    want = orc.dedispersion(inp, flags)
+    compare_rows(out.reshape(want.shape), want, w.obs.nr_samples_per_batch, "plain-128 patterns")
+
+
 def test_single_step_beam_subrange(torch_cuda, ragged_case):
     """firstSynthesizedBeam offsets the beam mapping; the output is relative
     to the launch (include/Dedispersion.hpp:228, 267)."""
@@ -112,13 +143,14 @@ def test_single_step_beam_subrange(torch_cuda, ragged_case):
     compare_rows(got, want[2:4], w.obs.nr_samples_per_batch, "beams 2..3")
 
 
-def test_single_step_arts(torch_cuda):
+@pytest.mark.parametrize("conf", [None, Conf(64, 8, 16, 4, 4, 3)], ids=["default", "patterns"])
+def test_single_step_arts(torch_cuda, conf):
     """ARTS-sized single step: 1536 channels, 2048 DMs, 24576 samples, 77
     zapped channels, Gaussian noise and 8 dispersed pulses.  The oracle checks
     96 DM rows across the range; the DM-0 row and the pulses are checked on
     every sample."""
     w = workloads.arts_single()
-    plan, inp, out, orc, flags = single_step(torch_cuda, w)
+    plan, inp, out, orc, flags = single_step(torch_cuda, w, conf=conf)
     D, S = w.obs.nr_dms, w.obs.nr_samples_per_batch
     out = out.reshape(1, D, plan.sizes.out_pitch)
     for r0 in (0, 1000, D - 32):

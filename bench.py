@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="batched-beams",
                     choices=["batched-beams", "arts-two-step", "arts-single"])
+    ap.add_argument("--input-bits", type=int, default=8, choices=[1, 2, 4, 8],
+                    help="bits per input sample (1, 2, 4 = packed rows of uniform random fields)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -90,20 +92,26 @@ class EventTimer:
         return {k: sum(a.elapsed_time(b) for a, b in v) / len(v) for k, v in self.pairs.items()}
 
 
-def setup_two_step(torch, w):
-    from dedispersion_amd import Plan, synth_gaussian_u8
+def setup_two_step(torch, w, bits=8):
+    from dedispersion_amd import Plan, synth_gaussian_u8, synth_uniform_u8
 
-    plan = Plan(w.obs, 128)
+    plan = Plan(w.obs, 128, input_bits=bits)
     flags = w.zapped_flags() + [0] * (plan.pad(w.obs.nr_channels, 4) - w.obs.nr_channels)
     plan.set_zapped_channels(flags)
     z = plan.sizes
     s = torch.cuda.current_stream().cuda_stream
     d_in = torch.zeros(z.sub_input_elements, dtype=torch.uint8, device="cuda")
-    synth_gaussian_u8(d_in.data_ptr(), w.obs.nr_beams * w.obs.nr_channels, z.nr_dispersed_samples_subbanding,
-                      z.sub_in_pitch, 128.0, 16.0, w.seed, s)
-    for p in w.pulses:
-        plan.inject_pulse(d_in.data_ptr(), z.sub_in_pitch, z.nr_dispersed_samples_subbanding, p.dm_value, p.t0,
-                          p.width, p.amplitude, s)
+    rows = w.obs.nr_beams * w.obs.nr_channels
+    if bits == 8:
+        synth_gaussian_u8(d_in.data_ptr(), rows, z.nr_dispersed_samples_subbanding, z.sub_in_pitch, 128.0, 16.0,
+                          w.seed, s)
+        for p in w.pulses:
+            plan.inject_pulse(d_in.data_ptr(), z.sub_in_pitch, z.nr_dispersed_samples_subbanding, p.dm_value, p.t0,
+                              p.width, p.amplitude, s)
+    else:
+        per = 8 // bits
+        synth_uniform_u8(d_in.data_ptr(), rows, -(-z.nr_dispersed_samples_subbanding // per), z.sub_in_pitch,
+                         w.seed, s)
     d_mid = torch.empty(z.intermediate_elements, dtype=torch.float32, device="cuda")
     d_out = torch.empty(z.sub_output_elements, dtype=torch.float32, device="cuda")
     o = w.obs
@@ -118,16 +126,21 @@ def setup_two_step(torch, w):
     return plan, kernels, units, (d_in, d_mid, d_out)
 
 
-def setup_single(torch, w):
-    from dedispersion_amd import Plan, synth_gaussian_u8
+def setup_single(torch, w, bits=8):
+    from dedispersion_amd import Plan, synth_gaussian_u8, synth_uniform_u8
 
-    plan = Plan(w.obs, 128)
+    plan = Plan(w.obs, 128, input_bits=bits)
     flags = w.zapped_flags() + [0] * (plan.pad(w.obs.nr_channels, 4) - w.obs.nr_channels)
     plan.set_zapped_channels(flags)
     z = plan.sizes
     s = torch.cuda.current_stream().cuda_stream
     d_in = torch.zeros(z.input_elements, dtype=torch.uint8, device="cuda")
-    synth_gaussian_u8(d_in.data_ptr(), w.obs.nr_channels, z.nr_dispersed_samples, z.in_pitch, 128.0, 16.0, w.seed, s)
+    if bits == 8:
+        synth_gaussian_u8(d_in.data_ptr(), w.obs.nr_channels, z.nr_dispersed_samples, z.in_pitch, 128.0, 16.0,
+                          w.seed, s)
+    else:
+        synth_uniform_u8(d_in.data_ptr(), w.obs.nr_channels, -(-z.nr_dispersed_samples // (8 // bits)), z.in_pitch,
+                         w.seed, s)
     d_out = torch.empty(z.output_elements, dtype=torch.float32, device="cuda")
     o = w.obs
     units = o.nr_synthesized_beams * o.nr_dms * z.nr_samples
@@ -136,14 +149,14 @@ def setup_single(torch, w):
     return plan, kernels, units, (d_in, d_out)
 
 
-def cpu_baseline(w, mode: str, seconds: float):
+def cpu_baseline(w, mode: str, seconds: float, bits: int = 8):
     """The oracle on the host cores over a bounded sample of the workload."""
     import numpy as np
 
     import oracle
 
     threads = max(1, min(16, os.cpu_count() or 1))
-    orc = oracle.Oracle(w.obs, 128, threads=threads)
+    orc = oracle.Oracle(w.obs, 128, threads=threads, input_bits=bits)
     o = w.obs
     rng = np.random.default_rng(0)
     if mode == "two-step":
@@ -203,7 +216,7 @@ def run_batched(args, torch):
     from dedispersion_amd.beams import BeamSet
 
     ws = workloads.batched_beams()
-    bs = BeamSet(torch, ws)
+    bs = BeamSet(torch, ws, input_bits=args.input_bits)
     info = bs.info()
     units = info["units"]
     log(f"batched beams: {info}")
@@ -254,6 +267,8 @@ def run_batched(args, torch):
     value = units / (ms / 1e3)
     dms = sorted({w.obs.nr_dms_subbanding * w.obs.nr_dms for w in ws})
     workload = f"batched-beams-{len(ws)}x{ws[0].obs.nr_channels}ch-two-step"
+    if args.input_bits != 8:
+        workload += f"-{args.input_bits}bit"
     confs = {names[m]: bs.batch.get_conf(m).print() for m in names}
     samples = sorted({w.obs.nr_samples_per_batch for w in ws})
     line = {
@@ -268,9 +283,10 @@ def run_batched(args, torch):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u8/f32",
-        "data": "synthetic (seeded Gaussian noise N(128,16) clipped to uint8 + 8 dispersed pulses per beam, "
-                "5% zapped channels per beam)",
-        "config": {"workload": workload, "beams": len(ws),
+        "data": ("synthetic (seeded Gaussian noise N(128,16) clipped to uint8 + 8 dispersed pulses per beam, "
+                 "5% zapped channels per beam)" if args.input_bits == 8 else
+                 f"synthetic (seeded uniform random {args.input_bits}-bit samples, packed, 5% zapped channels per beam)"),
+        "config": {"workload": workload, "beams": len(ws), "input_bits": args.input_bits,
                    "channels": ws[0].obs.nr_channels, "subbands": ws[0].obs.nr_subbands, "dms_per_beam": dms,
                    "samples_per_beam": samples, "units_per_step": units, "padding": 128,
                    "launch": (f"pipelined: {info['pipeline_chunks']} chunks on 2 streams, tile-list grids, HIP graph"
@@ -300,11 +316,11 @@ def run_single_plan(args, torch):
 
     if args.workload == "arts-two-step":
         w = workloads.arts_two_step()
-        plan, kernels, units, keep = setup_two_step(torch, w)
+        plan, kernels, units, keep = setup_two_step(torch, w, args.input_bits)
         mode = "two-step"
     else:
         w = workloads.arts_single()
-        plan, kernels, units, keep = setup_single(torch, w)
+        plan, kernels, units, keep = setup_single(torch, w, args.input_bits)
         mode = "single-step"
     torch.cuda.synchronize()
 
@@ -341,8 +357,11 @@ def run_single_plan(args, torch):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u8/f32" if mode == "two-step" else "u8",
-        "data": "synthetic (seeded Gaussian noise N(128,16) + 8 dispersed pulses, 5% zapped channels)",
-        "config": {"workload": w.name, "mode": mode, "beams": w.obs.nr_synthesized_beams,
+        "data": ("synthetic (seeded Gaussian noise N(128,16) + 8 dispersed pulses, 5% zapped channels)"
+                 if args.input_bits == 8 else
+                 f"synthetic (seeded uniform random {args.input_bits}-bit samples, packed, 5% zapped channels)"),
+        "config": {"workload": w.name if args.input_bits == 8 else f"{w.name}-{args.input_bits}bit", "mode": mode,
+                   "input_bits": args.input_bits, "beams": w.obs.nr_synthesized_beams,
                    "channels": w.obs.nr_channels, "samples": w.obs.nr_samples_per_batch,
                    "dms": units // w.obs.nr_samples_per_batch // w.obs.nr_synthesized_beams, "padding": 128},
         "hbm_roofline_frac_path": total_bytes / (ms / 1e3) / HBM_PEAK,
@@ -371,7 +390,7 @@ def main():
         line, w, mode = run_single_plan(args, torch)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        line["cpu_baseline"] = cpu_baseline(w, mode, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(w, mode, args.cpu_seconds, args.input_bits)
     print(json.dumps(line), flush=True)
 
 

@@ -17,9 +17,13 @@ from .workloads import PADDING, Workload
 
 class BeamSet:
     def __init__(self, torch, workloads: List[Workload], padding: int = PADDING, confs: Optional[dict] = None,
-                 pulses: bool = True):
+                 pulses: bool = True, input_bits: int = 8):
         """confs: {mode: Conf} for the batch; None = the tuned batch
-        configurations packaged with the library (else the defaults)."""
+        configurations packaged with the library (the tuned pipelined pair
+        and chunk count when there is one, else the best of each kind, else
+        the defaults).
+        input_bits 1, 2 or 4: packed input rows of uniform random fields
+        (the workloads' noise model and pulses apply to 8-bit input only)."""
         self.torch = torch
         self.workloads = workloads
         self.padding = padding
@@ -31,7 +35,7 @@ class BeamSet:
             o = w.obs
             if o.nr_dms_subbanding == 0:
                 raise ValueError(f"{w.name}: a beam set runs the two-step path")
-            plan = Plan(o, padding)
+            plan = Plan(o, padding, input_bits=input_bits)
             flags = [0] * plan.pad(o.nr_channels, 4)
             for c in w.zapped:
                 flags[c] = 1
@@ -39,12 +43,16 @@ class BeamSet:
             z = plan.sizes
             d_in = torch.zeros(z.sub_input_elements, dtype=torch.uint8, device="cuda")
             rows = o.nr_beams * o.nr_channels
-            if w.noise == "uniform":
+            if input_bits != 8:
+                per = 8 // input_bits
+                synth_uniform_u8(d_in.data_ptr(), rows, -(-z.nr_dispersed_samples_subbanding // per), z.sub_in_pitch,
+                                 w.seed, s)
+            elif w.noise == "uniform":
                 synth_uniform_u8(d_in.data_ptr(), rows, z.nr_dispersed_samples_subbanding, z.sub_in_pitch, w.seed, s)
             else:
                 synth_gaussian_u8(d_in.data_ptr(), rows, z.nr_dispersed_samples_subbanding, z.sub_in_pitch, 128.0,
                                   16.0, w.seed, s)
-            if pulses:
+            if pulses and input_bits == 8:
                 for p in w.pulses:
                     plan.inject_pulse(d_in.data_ptr(), z.sub_in_pitch, z.nr_dispersed_samples_subbanding, p.dm_value,
                                       p.t0, p.width, p.amplitude, s)
@@ -54,9 +62,14 @@ class BeamSet:
             self.outs.append(torch.empty(z.sub_output_elements, dtype=torch.float32, device="cuda"))
         self.batch = Batch()
         if confs is None:
-            from .tuner import load_batch_confs
+            from .tuner import load_batch_confs, load_batch_pipeline
 
-            confs = load_batch_confs()
+            pipe = load_batch_pipeline()
+            if pipe is not None:
+                confs = pipe["confs"]
+                self.batch.set_pipeline(pipe["chunks"])
+            else:
+                confs = load_batch_confs()
         if confs:
             for mode, conf in confs.items():
                 self.batch.set_conf(mode, conf)

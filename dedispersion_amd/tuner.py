@@ -250,32 +250,93 @@ def load_batch_confs(path: str = BATCH_TUNED_PATH) -> Dict[int, Conf]:
     return {MODES[k]: Conf(**v["conf"]) for k, v in d.get("confs", {}).items()}
 
 
-def tune_batch(path: Optional[str], iterations: int = 3) -> Dict:
-    """Exhaustive sweep of the staged configurations of both kinds of the
-    40-beam batch; the best of each is written to `path`."""
+def time_pipelined(beams, iterations: int = 5) -> float:
+    """Milliseconds per graph replay of the whole batch (pipelined or not,
+    as the batch is set)."""
+    import torch
+
+    beams.run(use_graph=True)  # builds queues and graph, warms up
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(iterations):
+        beams.run(use_graph=True)
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / iterations
+
+
+def pipeline_candidates(recs: List[Dict], top: int, small_lds: int) -> List[Dict]:
+    """The `top` fastest configurations of one kind alone, plus the
+    `small_lds` fastest among those staging at most 4 rows per barrier (a
+    smaller LDS footprint lets the other kind's workgroups share a CU)."""
+    valid = sorted((r for r in recs if r.get("valid")), key=lambda r: r["ms"])
+    out = valid[:top]
+    out += [r for r in valid if r["conf"]["unroll"] <= 4 and r not in out][:small_lds]
+    return out
+
+
+def tune_batch(path: Optional[str], iterations: int = 3, top: int = 6, small_lds: int = 4,
+               chunk_counts=(2, 3, 4, 6)) -> Dict:
+    """Tunes the 40-beam batch: an exhaustive sweep of the staged
+    configurations of each kernel kind alone (persistent launches), then the
+    pipelined graph over pairs of the best candidates of each kind and chunk
+    counts.  The best of each kind alone and the best pipelined pair are
+    written to `path`."""
     import torch
 
     from .beams import BeamSet
 
-    bs = BeamSet(torch, wl.batched_beams())
-    best = {}
+    bs = BeamSet(torch, wl.batched_beams(), confs={})
+    best, cands = {}, {}
     try:
+        bs.batch.set_pipeline(0)
         bs.run(use_graph=False)  # a valid intermediate for step two
         for name in ("step_one", "step_two"):
             t0 = time.time()
             recs = sweep_batch(bs, MODES[name], staged_space(), iterations)
             b = best_of(recs)
             best[name] = {"conf": {k: v for k, v in b["conf"].items()}, "ms": b["ms"], "swept": len(recs)}
+            cands[name] = pipeline_candidates(recs, top, small_lds)
             bs.set_conf(MODES[name], Conf(**b["conf"]))
             print(f"batch {name}: {len(recs)} configurations in {time.time() - t0:.1f}s; best {b['conf']} "
                   f"{b['ms']:.3f} ms", file=sys.stderr, flush=True)
+        t0 = time.time()
+        pipe = None
+        n = 0
+        for r1 in cands["step_one"]:
+            for r2 in cands["step_two"]:
+                bs.set_conf(DD_STEP_ONE, Conf(**r1["conf"]))
+                bs.set_conf(DD_STEP_TWO, Conf(**r2["conf"]))
+                for chunks in chunk_counts:
+                    bs.batch.set_pipeline(chunks)
+                    ms = time_pipelined(bs, iterations + 2)
+                    n += 1
+                    if pipe is None or ms < pipe["ms"]:
+                        pipe = {"step_one": r1["conf"], "step_two": r2["conf"], "chunks": chunks, "ms": ms}
+        pipe["swept"] = n
+        print(f"pipelined: {n} pairs x chunk counts in {time.time() - t0:.1f}s; best {pipe}", file=sys.stderr,
+              flush=True)
     finally:
         bs.close()
-    out = {"workload": "batched-beams-40x1536ch-two-step", "device": "MI355X", "confs": best}
+    out = {"workload": "batched-beams-40x1536ch-two-step", "device": "MI355X", "confs": best, "pipeline": pipe}
     if path:
         with open(path, "w") as f:
             json.dump(out, f, indent=1)
     return out
+
+
+def load_batch_pipeline(path: str = BATCH_TUNED_PATH) -> Optional[Dict]:
+    """The tuned pipelined setting: {"confs": {mode: Conf}, "chunks": n}, or
+    None when the file holds none."""
+    try:
+        with open(path) as f:
+            p = json.load(f).get("pipeline")
+    except (OSError, ValueError):
+        return None
+    if not p:
+        return None
+    return {"confs": {DD_STEP_ONE: Conf(**p["step_one"]), DD_STEP_TWO: Conf(**p["step_two"])},
+            "chunks": int(p["chunks"])}
 
 
 def learned_tune(bench: Bench, model, budget: float, iterations: int) -> Dict:

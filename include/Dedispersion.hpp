@@ -115,6 +115,8 @@ public:
     check(dd_plan_get_sizes(plan_.get(), &s));
     return s;
   }
+  // The reference's inputBits: 1, 2, 4 (packed input rows) or 8.
+  void setInputBits(unsigned int inputBits) { check(dd_plan_set_input_bits(plan_.get(), inputBits)); }
   void setConf(dd_mode mode, const DedispersionConf &conf) {
     dd_conf c = conf.toC();
     check(dd_plan_set_conf(plan_.get(), mode, &c));
@@ -169,14 +171,19 @@ public:
 
 // Factories with the generators' argument lists (include/Dedispersion.hpp:63-65).
 // I must be uint8_t and O float (include/configuration.hpp:9-15); inputBits
-// below 8 is not supported yet and throws.
+// is 1, 2, 4 (packed rows, see dd_plan_set_input_bits) or 8, else throws.
 template <typename I, typename O>
 DedispersionKernel *getDedispersionHIP(const DedispersionConf &conf, const unsigned int padding,
                                        const uint8_t inputBits, const AstroData::Observation &observation,
                                        std::vector<float> &shifts) {
   static_assert(sizeof(I) == 1 && sizeof(O) == 4, "uint8 input and float output only");
-  if (inputBits != 8) throw HIPError(DD_ERR_UNSUPPORTED, "inputBits below 8 is not supported");
   DedispersionKernel *k = new DedispersionKernel(observation, padding);
+  try {
+    k->setInputBits(inputBits);
+  } catch (...) {
+    delete k;
+    throw;
+  }
   k->setShifts(&shifts, nullptr);
   k->setConf(DD_SINGLE_STEP, conf);
   return k;
@@ -188,8 +195,13 @@ SubbandStepOneKernel *getSubbandDedispersionStepOneHIP(const DedispersionConf &c
                                                        const AstroData::Observation &observation,
                                                        std::vector<float> &shifts) {
   static_assert(sizeof(I) == 1 && sizeof(O) == 4, "uint8 input and float output only");
-  if (inputBits != 8) throw HIPError(DD_ERR_UNSUPPORTED, "inputBits below 8 is not supported");
   SubbandStepOneKernel *k = new SubbandStepOneKernel(observation, padding);
+  try {
+    k->setInputBits(inputBits);
+  } catch (...) {
+    delete k;
+    throw;
+  }
   k->setShifts(&shifts, nullptr);
   k->setConf(DD_STEP_ONE, conf);
   return k;

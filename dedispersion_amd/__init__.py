@@ -147,6 +147,17 @@ class Plan:
         c = conf.to_c()
         check(self.lib.dd_plan_set_conf(self.handle, mode, ctypes.byref(c)))
 
+    def apply_tuned(self, mode: int, tuned: dict, device: str = "MI355X") -> Conf:
+        """Sets the configuration of `mode` from a tuned-configuration map
+        (tuner.read_tuned_file, the reference's tunedDedispersionConf) at
+        this plan's DM count for the mode: D for the single step and step
+        two, Dc for step one.  KeyError when the map has no entry."""
+        o = self.obs
+        nr_dms = o.nr_dms_subbanding if mode == DD_STEP_ONE else o.nr_dms
+        conf = tuned[device][nr_dms]
+        self.set_conf(mode, conf)
+        return conf
+
     def get_conf(self, mode: int) -> Conf:
         c = _lib.DDConf()
         check(self.lib.dd_plan_get_conf(self.handle, mode, ctypes.byref(c)))

@@ -60,7 +60,7 @@ std::string KernelConf::print() const {
          std::to_string(nrItemsD0) + " " + std::to_string(nrItemsD1) + " " + std::to_string(nrItemsD2);
 }
 
-DedispersionConf::DedispersionConf() : KernelConf(), splitBatches(false), local(false), unroll(1) {}
+DedispersionConf::DedispersionConf() : KernelConf(), splitBatches(false), local(0), unroll(1) {}
 
 DedispersionConf::~DedispersionConf() {}
 
@@ -76,7 +76,7 @@ dd_conf DedispersionConf::toC() const {
   c.items_d0 = getNrItemsD0();
   c.items_d1 = getNrItemsD1();
   c.unroll = unroll;
-  c.local = local ? 1 : 0;
+  c.local = local;
   c.split_batches = splitBatches ? 1 : 0;
   return c;
 }
@@ -88,7 +88,7 @@ DedispersionConf DedispersionConf::fromC(const dd_conf &c) {
   conf.setNrItemsD0(c.items_d0);
   conf.setNrItemsD1(c.items_d1);
   conf.setUnroll(c.unroll);
-  conf.setLocalMem(c.local != 0);
+  conf.setLocalMode(c.local);
   conf.setSplitBatches(c.split_batches != 0);
   return conf;
 }
@@ -104,12 +104,12 @@ void readTunedDedispersionConf(tunedDedispersionConf &tunedDedispersion, const s
     if (line.empty() || !std::isalpha(static_cast<unsigned char>(line[0]))) continue;
     std::istringstream fields(line);
     std::string device;
-    unsigned int nrDMs = 0, t0 = 0, t1 = 0, t2 = 0, i0 = 0, i1 = 0, i2 = 0, unroll = 0;
-    bool split = false, local = false;
+    unsigned int nrDMs = 0, local = 0, t0 = 0, t1 = 0, t2 = 0, i0 = 0, i1 = 0, i2 = 0, unroll = 0;
+    bool split = false;
     fields >> device >> nrDMs >> split >> local >> unroll >> t0 >> t1 >> t2 >> i0 >> i1 >> i2;
     auto *conf = new DedispersionConf();
     conf->setSplitBatches(split);
-    conf->setLocalMem(local);
+    conf->setLocalMode(local);
     conf->setUnroll(unroll);
     conf->setNrThreadsD0(t0);
     conf->setNrThreadsD1(t1);
@@ -121,8 +121,8 @@ void readTunedDedispersionConf(tunedDedispersionConf &tunedDedispersion, const s
       auto *container = new std::map<unsigned int, DedispersionConf *>();
       container->insert(std::make_pair(nrDMs, conf));
       tunedDedispersion.insert(std::make_pair(device, container));
-    } else {
-      tunedDedispersion.at(device)->insert(std::make_pair(nrDMs, conf));
+    } else if (!tunedDedispersion.at(device)->insert(std::make_pair(nrDMs, conf)).second) {
+      delete conf;  // the first entry for (device, nrDMs) stays
     }
   }
 }

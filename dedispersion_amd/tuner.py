@@ -192,6 +192,64 @@ def write_tuned_file(path: str, entries: List[str]) -> None:
             f.write(e + "\n")
 
 
+def read_tuned_file(path: str) -> Dict[str, Dict[int, Conf]]:
+    """The reference's reader (src/Dedispersion.cpp:20-81): lines starting
+    with a letter are ``device nrDMs split local unroll t0 t1 t2 i0 i1 i2``;
+    others (comments, blank lines) are skipped; the first line for a
+    (device, nrDMs) is kept (std::map::insert, :71-78).  local is the local
+    mode 0-3.  Raises OSError if the file cannot be opened (the reference
+    throws AstroData::FileError)."""
+    out: Dict[str, Dict[int, Conf]] = {}
+    with open(path) as f:
+        for line in f:
+            if not line[:1].isalpha():
+                continue
+            v = line.split()
+            split, local, unroll, t0, t1, _t2, i0, i1, _i2 = (int(x) for x in v[2:11])
+            out.setdefault(v[0], {}).setdefault(int(v[1]), Conf(t0, t1, i0, i1, unroll, local, split))
+    return out
+
+
+def tuned_lines_from_records(records: List[Dict], device: str = "MI355X") -> Dict[str, List[str]]:
+    """Reference-format lines per kernel kind from sweep records: for each
+    DM count, the configuration with the least summed time relative to each
+    layout's best, over the layouts with that DM count that timed it."""
+    from . import learned
+
+    out: Dict[str, List[str]] = {}
+    groups: Dict[tuple, Dict[str, List[Dict]]] = {}
+    for key, recs in learned.by_layout(records).items():
+        lay = recs[0]["layout"]
+        groups.setdefault((lay["mode"], lay["dms"]), {})[key] = recs
+    for (mode, dms), layouts in sorted(groups.items()):
+        score: Dict[tuple, float] = {}
+        count: Dict[tuple, int] = {}
+        for recs in layouts.values():
+            best = min(r["ms"] for r in recs)
+            for r in recs:
+                c = tuple(r["conf"])
+                score[c] = score.get(c, 0.0) + r["ms"] / best
+                count[c] = count.get(c, 0) + 1
+        full = [c for c in score if count[c] == len(layouts)]
+        pick = min(full, key=lambda c: score[c])
+        out.setdefault(mode, []).append(tuned_conf_line(device, dms, asdict(Conf(*pick))))
+    return out
+
+
+TUNED_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
+
+
+def tuned_file_path(mode: str, device: str = "MI355X") -> str:
+    """Packaged reference-format file of one kernel kind (mode name as in MODES)."""
+    return os.path.join(TUNED_DIR, f"{mode}_{device.lower()}.conf")
+
+
+def select_tuned_conf(tuned: Dict[str, Dict[int, Conf]], device: str, nr_dms: int) -> Conf:
+    """tuned[device][nrDMs], the lookup a pipeline makes in the reader's map;
+    KeyError when either is absent."""
+    return tuned[device][nr_dms]
+
+
 def _beam_shapes():
     """Layouts of the 40-beam workload (two-step, coarse DMs x 32 fine DMs of
     0.2, samples per beam), held out of the learned tuner's training set."""
@@ -362,7 +420,17 @@ def main(argv=None):
                     help="train the cost model on the recorded sweeps and time only the best-predicted fraction")
     ap.add_argument("--budget", type=float, default=0.1, help="fraction of the space timed with --learned")
     ap.add_argument("--train-records", default=None, help="sweep records for --learned (default: the packaged ones)")
+    ap.add_argument("--from-records", default=None, metavar="RECORDS",
+                    help="write the reference-format files of every kernel kind from sweep records (no device)")
     args = ap.parse_args(argv)
+    if args.from_records:
+        from . import learned
+
+        for mode, lines in tuned_lines_from_records(learned.load_records(args.from_records), args.device).items():
+            path = tuned_file_path(mode, args.device)
+            write_tuned_file(path, lines)
+            print(f"{path}: {len(lines)} entries", file=sys.stderr)
+        return
     import torch
 
     if args.workload == "batched-beams":

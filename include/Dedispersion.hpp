@@ -70,12 +70,18 @@ public:
   DedispersionConf();
   ~DedispersionConf();
   bool getSplitBatches() const { return splitBatches; }
-  bool getLocalMem() const { return local; }
+  bool getLocalMem() const { return local != 0; }
   unsigned int getUnroll() const { return unroll; }
   void setSplitBatches(bool split) { splitBatches = split; }
-  void setLocalMem(bool local) { this->local = local; }
+  void setLocalMem(bool local) { this->local = local ? 1 : 0; }
   void setUnroll(unsigned int unroll) { this->unroll = unroll; }
-  // "split local unroll t0 t1 t2 i0 i1 i2" (src/Dedispersion.cpp:88)
+  // The kernel selection behind the local flag: dd_conf.local (0 = caches,
+  // 1 = pipelined LDS staging, 2 = LDS staging one group at a time,
+  // 3 = channel-block patterns); getLocalMem() is true for 1, 2 and 3.
+  unsigned int getLocalMode() const { return local; }
+  void setLocalMode(unsigned int mode) { local = mode; }
+  // "split local unroll t0 t1 t2 i0 i1 i2" (src/Dedispersion.cpp:88); local
+  // is the local mode.
   std::string print() const;
 
   dd_conf toC() const;
@@ -83,7 +89,7 @@ public:
 
 private:
   bool splitBatches;
-  bool local;
+  unsigned int local;
   unsigned int unroll;
 };
 
@@ -92,8 +98,10 @@ typedef std::map<std::string, std::map<unsigned int, Dedispersion::DedispersionC
 
 // src/Dedispersion.cpp:20-81: lines starting with a letter are
 // "device nrDMs split local unroll t0 t1 t2 i0 i1 i2"; entries are heap
-// allocated and owned by the caller.  Throws AstroData::FileError if the
-// file cannot be opened.
+// allocated and owned by the caller; the first line for a (device, nrDMs)
+// is kept (later ones are dropped, which the reference leaks).  local is read
+// as the local mode (0-3; the reference reads a bool).  Throws
+// AstroData::FileError if the file cannot be opened.
 void readTunedDedispersionConf(tunedDedispersionConf &tunedDedispersion, const std::string &dedispersionFilename);
 
 // Observation -> C descriptor (dispersed sizes taken from the observation

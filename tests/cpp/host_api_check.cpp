@@ -60,6 +60,19 @@ int main(int argc, char **argv) {
     }
     return 0;
   }
+  if (mode == "readconf") {
+    // args: file; prints every entry the reader returns
+    Dedispersion::tunedDedispersionConf tuned;
+    Dedispersion::readTunedDedispersionConf(tuned, argv[2]);
+    for (auto &dev : tuned) {
+      for (auto &e : *dev.second) {
+        std::printf("%s %u %s\n", dev.first.c_str(), e.first, e.second->print().c_str());
+        delete e.second;
+      }
+      delete dev.second;
+    }
+    return 0;
+  }
   if (mode == "zapped") {
     AstroData::Observation obs;
     obs.setFrequencyRange(1, 16, 1400.0f, 1.0f);

@@ -22,9 +22,10 @@
 //  * Up to four rows are added as 16-bit pairs (exact while four subband
 //    sums stay below 2^16), then each half of the result is added to its own
 //    32-bit sum (SDWA adds, v_add_u32 with src0_sel WORD_0 / WORD_1).
-//  * The staging lanes store each DM's final dword offset (copy select and
-//    position) in the row's metadata; the summing lanes read it as a
-//    broadcast into a VGPR, so no per-DM scalar work remains.
+//  * The staging lanes store each DM's LDS byte address (row window, copy
+//    select and position) in the row's metadata; the summing lanes read it
+//    as a broadcast into a VGPR and add their lane offset, one add per row
+//    and DM.
 #include "staged_common.hpp"
 
 namespace ddamd {
@@ -128,7 +129,8 @@ __device__ __forceinline__ void u16_tile(const RowSumF32Params &p, uint32_t logi
             *reinterpret_cast<u32x4 *>(w1 + 4 * m) = r;
           }
         }
-        if (lane < TD) meta[(buf * G + g) * TD + lane] = so[s];
+        // the DM's LDS byte address (window copy and position) for the summing lanes
+        if (lane < TD) meta[(buf * G + g) * TD + lane] = 4 * (rows_pad + ((buf * G + g) * 2) * WD + so[s]);
       }
     }
   };
@@ -142,14 +144,16 @@ __device__ __forceinline__ void u16_tile(const RowSumF32Params &p, uint32_t logi
   // R rows are first added as 16-bit pairs (R x the largest subband sum
   // stays below 2^16: p.combine_rows >= R), then each half goes to its
   // 32-bit sum with one SDWA add.
+  // Per-lane base of the LDS addresses: a DM's address is its metadata word
+  // (a broadcast read) plus this, one add per row and DM.
+  const char *lane_base = reinterpret_cast<const char *>(lds) + 4 * lane;
   auto rows = [&](auto R, uint32_t buf, uint32_t g) {
     const uint32_t *m = meta + (buf * G + g) * TD + dw;  // broadcast reads
-    const uint32_t *w = windows + ((buf * G + g) * 2) * WD + lane;
 #pragma unroll
     for (int j = 0; j < DPW; ++j) {
       const uint32_t *src[R.value];
 #pragma unroll
-      for (int r = 0; r < R.value; ++r) src[r] = w + r * 2 * WD + m[r * TD + j];
+      for (int r = 0; r < R.value; ++r) src[r] = reinterpret_cast<const uint32_t *>(lane_base + m[r * TD + j]);
 #pragma unroll
       for (int k = 0; k < NP; ++k) {
         uint32_t x = src[0][64 * k];

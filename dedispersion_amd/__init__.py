@@ -280,6 +280,9 @@ class Batch:
     def set_pipeline(self, chunks: int) -> None:
         check(self.lib.dd_batch_set_pipeline(self.handle, chunks))
 
+    def set_lds_reserve(self, mode: int, nbytes: int) -> None:
+        check(self.lib.dd_batch_set_lds_reserve(self.handle, mode, nbytes))
+
     def set_compact_intermediate(self, enable: bool) -> None:
         check(self.lib.dd_batch_set_compact_intermediate(self.handle, 1 if enable else 0))
 

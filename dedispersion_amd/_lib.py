@@ -159,6 +159,7 @@ PROTOTYPES = {
     "dd_batch_get_conf": (c_int, [c_void_p, c_int, POINTER(DDConf)]),
     "dd_batch_set_compact_intermediate": (c_int, [c_void_p, c_int]),
     "dd_batch_set_pipeline": (c_int, [c_void_p, c_uint32]),
+    "dd_batch_set_lds_reserve": (c_int, [c_void_p, c_int, c_uint32]),
     "dd_batch_run": (c_int, [c_void_p, c_int, c_void_p]),
     "dd_batch_run_mode": (c_int, [c_void_p, c_int, c_void_p]),
     "dd_batch_run_per_job": (c_int, [c_void_p, c_void_p]),

@@ -68,6 +68,8 @@ class BeamSet:
             if pipe is not None:
                 confs = pipe["confs"]
                 self.batch.set_pipeline(pipe["chunks"])
+                for mode, nbytes in pipe["lds_reserve"].items():
+                    self.batch.set_lds_reserve(mode, nbytes)
             else:
                 confs = load_batch_confs()
         if confs:

@@ -85,6 +85,7 @@ struct dd_batch {
   hipGraphExec_t exec = nullptr;
   // pipelined mode
   uint32_t chunks = 3;                         // <= 1: one persistent launch per kind
+  uint32_t lds_reserve[3] = {0, 0, 0};         // least LDS bytes per workgroup, per kind
   std::vector<uint32_t> off1, n1, off2, n2;    // per chunk: tile list offsets and counts
   DeviceBuffer<TileDesc> d_list1, d_list2;     // step one / step two tiles, chunk after chunk
   hipStream_t side[2] = {nullptr, nullptr};    // step one, step two
@@ -241,7 +242,8 @@ dd_status build(dd_batch *b) {
     if (tiles.size() > 0xFFFFFFF0u) return fail(DD_ERR_UNSUPPORTED, "too many tiles in one batch");
     k.n_tiles = (uint32_t)tiles.size();
     if (k.n_tiles == 0) continue;
-    if (k.g.lds_bytes > b->jobs.front().plan->max_lds)
+    // the persistent kernels also hold 16 bytes of static LDS (the claim slot, aligned)
+    if (k.g.lds_bytes + 16 > b->jobs.front().plan->max_lds || b->lds_reserve[m] > b->jobs.front().plan->max_lds)
       return fail(DD_ERR_CONFIGURATION, "batch LDS demand exceeds the device limit");
     int per_cu = 0;
     if (m == DD_STEP_TWO) {
@@ -327,15 +329,19 @@ dd_status enqueue_pipelined(dd_batch *b, hipStream_t s) {
   dd_status st = enqueue_kind(b, DD_SINGLE_STEP, a);
   if (st != DD_OK) return st;
   const KindQueue &k1 = b->q[DD_STEP_ONE], &k2 = b->q[DD_STEP_TWO];
+  // the pipelined launches request the LDS reserves (room for the other kind)
+  Geometry g1 = k1.g, g2 = k2.g;
+  g1.lds_bytes = std::max(g1.lds_bytes, b->lds_reserve[DD_STEP_ONE]);
+  g2.lds_bytes = std::max(g2.lds_bytes, b->lds_reserve[DD_STEP_TWO]);
   for (size_t c = 0; c < b->n1.size(); ++c) {
     if (b->n1[c])
-      DD_HIP(launch_rowsum_u8_list(k1.d_u8.ptr, b->d_list1.ptr + b->off1[c], b->n1[c], k1.g, k1.flush, a));
+      DD_HIP(launch_rowsum_u8_list(k1.d_u8.ptr, b->d_list1.ptr + b->off1[c], b->n1[c], g1, k1.flush, a));
     DD_HIP(hipEventRecord(b->events[3 + c], a));
     DD_HIP(hipStreamWaitEvent(c2, b->events[3 + c], 0));
     if (b->n2[c] && b->compact)
-      DD_HIP(launch_rowsum_u16_list(k2.d_f32.ptr, b->d_list2.ptr + b->off2[c], b->n2[c], k2.g, c2));
+      DD_HIP(launch_rowsum_u16_list(k2.d_f32.ptr, b->d_list2.ptr + b->off2[c], b->n2[c], g2, c2));
     else if (b->n2[c])
-      DD_HIP(launch_rowsum_f32_list(k2.d_f32.ptr, b->d_list2.ptr + b->off2[c], b->n2[c], k2.g, c2));
+      DD_HIP(launch_rowsum_f32_list(k2.d_f32.ptr, b->d_list2.ptr + b->off2[c], b->n2[c], g2, c2));
   }
   DD_HIP(hipEventRecord(join_a, a));
   DD_HIP(hipEventRecord(join_b, c2));
@@ -460,6 +466,13 @@ dd_status dd_batch_set_pipeline(dd_batch *batch, uint32_t chunks) {
   if (!batch) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
   if (chunks > 64) return fail(DD_ERR_INVALID_ARGUMENT, "at most 64 pipeline chunks");
   batch->chunks = chunks;
+  batch->invalidate();
+  return DD_OK;
+}
+
+dd_status dd_batch_set_lds_reserve(dd_batch *batch, dd_mode mode, uint32_t bytes) {
+  if (!batch || (int)mode < 0 || (int)mode > 2) return fail(DD_ERR_INVALID_ARGUMENT, "invalid argument");
+  batch->lds_reserve[mode] = bytes;
   batch->invalidate();
   return DD_OK;
 }

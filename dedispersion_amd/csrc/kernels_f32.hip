@@ -34,7 +34,7 @@ __device__ __forceinline__ uint32_t records_from(uint64_t total_floats, uint64_t
 template <int RUN, int DPW, bool LDS>
 __global__ void __launch_bounds__(512) rowsum_f32(const RowSumF32Params p) {
   constexpr int TS = 64 * RUN;
-  extern __shared__ float ldsf[];
+  extern __shared__ __attribute__((aligned(16))) float ldsf[];
 
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t wave = uniform(threadIdx.x >> 6);

@@ -154,7 +154,7 @@ __device__ __forceinline__ void f32_tile(const RowSumF32Params &p, uint32_t logi
 
 template <int RUN, int DPW>
 __global__ void __launch_bounds__(512) rowsum_f32_staged(const RowSumF32Params p) {
-  extern __shared__ float ldsf[];
+  extern __shared__ __attribute__((aligned(16))) float ldsf[];
   f32_tile<RUN, DPW>(p, xcd_logical(blockIdx.x, p.n_tiles), ldsf);
 }
 
@@ -162,7 +162,7 @@ __global__ void __launch_bounds__(512) rowsum_f32_staged(const RowSumF32Params p
 template <int RUN, int DPW>
 __global__ void __launch_bounds__(512) rowsum_f32_batch(const RowSumF32Params *__restrict__ beams,
                                                           const TileQueues q) {
-  extern __shared__ float ldsf[];
+  extern __shared__ __attribute__((aligned(16))) float ldsf[];
   __shared__ uint32_t claimed;
   QueueCursor cur;
   for (;;) {
@@ -177,7 +177,7 @@ __global__ void __launch_bounds__(512) rowsum_f32_batch(const RowSumF32Params *_
 template <int RUN, int DPW>
 __global__ void __launch_bounds__(512) rowsum_f32_list(const RowSumF32Params *__restrict__ jobs, const TileDesc *__restrict__ tiles,
                                                       uint32_t n) {
-  extern __shared__ float ldsf[];
+  extern __shared__ __attribute__((aligned(16))) float ldsf[];
   const TileDesc d = tiles[xcd_logical(blockIdx.x, n)];
   f32_tile<RUN, DPW>(jobs[d.job], d.tile, ldsf);
 }

@@ -11,21 +11,21 @@
 // The pipeline is that of kernels_f32_staged.hip (double-buffered subband
 // staging, shifts from the float delay table published through LDS,
 // XCD-aware order).  What changes:
-//  * Lane l owns the sample pairs t0 + 2l + 128k; one dword holds a pair, so
-//    a DM's run of 64 pairs is one conflict-free dword per lane and the LDS
-//    reads per sample are half those of float rows.
-//  * A DM's offset into the window may be odd; every staged row is kept in
-//    two copies, the second shifted by one element, so all reads are dword
-//    aligned (misaligned LDS reads are correct on gfx950 but several times
-//    slower).  The shifted copy is built in registers with v_alignbit from
-//    the 16-byte chunk plus one more dword.
+//  * Lane l owns the sample quads t0 + 4l + 256k; a quad is one ds_read_b64,
+//    so a DM's run of 64 quads is one conflict-free 8-byte read per lane and
+//    the LDS reads per sample are a quarter of those of float rows.
+//  * A DM's offset into the window is any element; every staged row is kept
+//    in four copies, copy c shifted down by c elements, so all reads are
+//    8-byte aligned (misaligned LDS reads are correct on gfx950 but several
+//    times slower).  The shifted copies are built in registers with
+//    v_alignbit from the 16-byte chunk plus the next two dwords.
 //  * Up to four rows are added as 16-bit pairs (exact while four subband
 //    sums stay below 2^16), then each half of the result is added to its own
 //    32-bit sum (SDWA adds, v_add_u32 with src0_sel WORD_0 / WORD_1).
 //  * The staging lanes store each DM's LDS byte address (row window, copy
-//    select and position) in the row's metadata; the summing lanes read it
-//    as a broadcast into a VGPR and add their lane offset, one add per row
-//    and DM.
+//    and position) in the row's metadata; the summing lanes read it as a
+//    broadcast into a VGPR and add their lane offset, one add per row and
+//    DM.
 #include "staged_common.hpp"
 
 namespace ddamd {
@@ -35,11 +35,11 @@ using namespace staged;
 
 template <int RUN, int DPW>
 __device__ __forceinline__ void u16_tile(const RowSumF32Params &p, uint32_t logical, uint32_t *lds) {
-  constexpr int NP = RUN / 2;   // sample pairs per lane
+  constexpr int NQ = RUN / 4;   // sample quads per lane
   constexpr int TS = 64 * RUN;  // samples per tile
   constexpr int SPW = kStagedSlotsPerWave;
   constexpr int CPL = RUN / 8 + 1;  // 16-byte staging loads per lane per row
-  typedef float f32x2 __attribute__((ext_vector_type(2)));
+  typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t wave = uniform(threadIdx.x >> 6);
@@ -59,11 +59,11 @@ __device__ __forceinline__ void u16_tile(const RowSumF32Params &p, uint32_t logi
   const uint32_t rows_pad = (2 * p.n_rows + 3) & ~3u;
   const uint16_t *in16 = reinterpret_cast<const uint16_t *>(p.in);
 
-  // LDS: rows (input beam, float delay) | windows [2][G][2 copies][WD] | shifts [2][G][TD]
+  // LDS: rows (input beam, float delay) | windows [2][G][4 copies][WD] | DM addresses [2][G][TD]
   uint32_t *lbeam = lds;
   float *ltab = reinterpret_cast<float *>(lds + p.n_rows);
   uint32_t *windows = lds + rows_pad;
-  uint32_t *meta = windows + 2 * G * 2 * WD;
+  uint32_t *meta = windows + 2 * G * 4 * WD;
 
   for (uint32_t sb = threadIdx.x; sb < p.n_rows; sb += blockDim.x) {
     lbeam[sb] = p.map[(uint64_t)(p.first_sbeam + sbeam) * p.map_pitch + sb];
@@ -72,7 +72,7 @@ __device__ __forceinline__ void u16_tile(const RowSumF32Params &p, uint32_t logi
   __syncthreads();
 
   u32x4 sv[SPW][CPL];
-  uint32_t sx[SPW][CPL];
+  u32x2 sx[SPW][CPL];
   uint32_t so[SPW];
   auto stage_load = [&](uint32_t s0, uint32_t nn) {
 #pragma unroll
@@ -88,79 +88,95 @@ __device__ __forceinline__ void u16_tile(const RowSumF32Params &p, uint32_t logi
                                        (uint64_t)sb * p.in_row_pitch + t0 + shmin);
         const uint64_t a0 = off & ~uint64_t(7);  // 16-byte aligned window start
         const uint32_t o = sh - shmin + (uint32_t)(off - a0);  // element offset in the window
-        so[s] = (o & 1) * WD + (o >> 1);                          // dword offset: copy (o & 1), pair o >> 1
+        // byte offset in the row's four copies: copy (o & 3), quad o >> 2
+        so[s] = (o & 3) * 4 * WD + 8 * (o >> 2);
         const uint32_t recs = records_from(2 * p.in_floats, 2 * a0);
         const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(in16 + a0, recs);
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
           const uint32_t m = lane + 64 * c;
           if (m < NCH) {
-            if (16 * m + 20 <= recs) {
+            if (16 * m + 24 <= recs) {
               sv[s][c] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, 16 * m, 0, 0);
-              sx[s][c] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, 16 * m + 16, 0, 0);
+              sx[s][c] = __builtin_amdgcn_raw_buffer_load_b64(rsrc, 16 * m + 16, 0, 0);
             } else {
 #pragma unroll
               for (int e = 0; e < 4; ++e) sv[s][c][e] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, 16 * m + 4 * e, 0, 0);
-              sx[s][c] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, 16 * m + 16, 0, 0);
+              sx[s][c][0] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, 16 * m + 16, 0, 0);
+              sx[s][c][1] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, 16 * m + 20, 0, 0);
             }
           }
         }
       }
     }
   };
+  // Copy c of a row holds the window shifted down by c elements, so the quad
+  // at any element offset o starts 8-byte aligned in copy o & 3.
   auto stage_store = [&](uint32_t buf, uint32_t nn) {
 #pragma unroll
     for (int s = 0; s < SPW; ++s) {
       const uint32_t g = wave + s * nwaves;
       if (g < nn) {
-        uint32_t *w0 = windows + ((buf * G + g) * 2) * WD;
-        uint32_t *w1 = w0 + WD;
+        uint32_t *w0 = windows + ((buf * G + g) * 4) * WD;
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
           const uint32_t m = lane + 64 * c;
           if (m < NCH) {
             const u32x4 v = sv[s][c];
-            u32x4 r;
-            r[0] = __builtin_amdgcn_alignbit(v[1], v[0], 16);
-            r[1] = __builtin_amdgcn_alignbit(v[2], v[1], 16);
-            r[2] = __builtin_amdgcn_alignbit(v[3], v[2], 16);
-            r[3] = __builtin_amdgcn_alignbit(sx[s][c], v[3], 16);
+            const uint32_t x0 = sx[s][c][0], x1 = sx[s][c][1];
+            u32x4 r1, r2, r3;
+            r1[0] = __builtin_amdgcn_alignbit(v[1], v[0], 16);
+            r1[1] = __builtin_amdgcn_alignbit(v[2], v[1], 16);
+            r1[2] = __builtin_amdgcn_alignbit(v[3], v[2], 16);
+            r1[3] = __builtin_amdgcn_alignbit(x0, v[3], 16);
+            r2[0] = v[1];
+            r2[1] = v[2];
+            r2[2] = v[3];
+            r2[3] = x0;
+            r3[0] = r1[1];
+            r3[1] = r1[2];
+            r3[2] = r1[3];
+            r3[3] = __builtin_amdgcn_alignbit(x1, x0, 16);
             *reinterpret_cast<u32x4 *>(w0 + 4 * m) = v;
-            *reinterpret_cast<u32x4 *>(w1 + 4 * m) = r;
+            *reinterpret_cast<u32x4 *>(w0 + WD + 4 * m) = r1;
+            *reinterpret_cast<u32x4 *>(w0 + 2 * WD + 4 * m) = r2;
+            *reinterpret_cast<u32x4 *>(w0 + 3 * WD + 4 * m) = r3;
           }
         }
-        // the DM's LDS byte address (window copy and position) for the summing lanes
-        if (lane < TD) meta[(buf * G + g) * TD + lane] = 4 * (rows_pad + ((buf * G + g) * 2) * WD + so[s]);
+        // the DM's LDS byte address for the summing lanes
+        if (lane < TD) meta[(buf * G + g) * TD + lane] = 4 * (rows_pad + ((buf * G + g) * 4) * WD) + so[s];
       }
     }
   };
 
-  uint32_t lo[DPW][NP], hi[DPW][NP];
+  uint32_t lo[DPW][2 * NQ], hi[DPW][2 * NQ];
 #pragma unroll
   for (int j = 0; j < DPW; ++j)
 #pragma unroll
-    for (int k = 0; k < NP; ++k) lo[j][k] = hi[j][k] = 0;
+    for (int k = 0; k < 2 * NQ; ++k) lo[j][k] = hi[j][k] = 0;
 
   // R rows are first added as 16-bit pairs (R x the largest subband sum
   // stays below 2^16: p.combine_rows >= R), then each half goes to its
-  // 32-bit sum with one SDWA add.
-  // Per-lane base of the LDS addresses: a DM's address is its metadata word
-  // (a broadcast read) plus this, one add per row and DM.
-  const char *lane_base = reinterpret_cast<const char *>(lds) + 4 * lane;
+  // 32-bit sum with one SDWA add.  A DM's address is its metadata word (a
+  // broadcast read) plus the lane offset, one add per row and DM; one
+  // ds_read_b64 fetches a quad.
+  const char *lane_base = reinterpret_cast<const char *>(lds) + 8 * lane;
   auto rows = [&](auto R, uint32_t buf, uint32_t g) {
     const uint32_t *m = meta + (buf * G + g) * TD + dw;  // broadcast reads
 #pragma unroll
     for (int j = 0; j < DPW; ++j) {
-      const uint32_t *src[R.value];
+      const u32x2 *src[R.value];
 #pragma unroll
-      for (int r = 0; r < R.value; ++r) src[r] = reinterpret_cast<const uint32_t *>(lane_base + m[r * TD + j]);
+      for (int r = 0; r < R.value; ++r) src[r] = reinterpret_cast<const u32x2 *>(lane_base + m[r * TD + j]);
 #pragma unroll
-      for (int k = 0; k < NP; ++k) {
-        uint32_t x = src[0][64 * k];
+      for (int k = 0; k < NQ; ++k) {
+        u32x2 x = src[0][64 * k];
 #pragma unroll
         for (int r = 1; r < R.value; ++r) x += src[r][64 * k];
-        lo[j][k] += x & 0xFFFFu;
-        hi[j][k] += x >> 16;
+        lo[j][2 * k] += x[0] & 0xFFFFu;
+        hi[j][2 * k] += x[0] >> 16;
+        lo[j][2 * k + 1] += x[1] & 0xFFFFu;
+        hi[j][2 * k + 1] += x[1] >> 16;
       }
     }
   };
@@ -198,30 +214,41 @@ __device__ __forceinline__ void u16_tile(const RowSumF32Params &p, uint32_t logi
     if (dm >= p.n_dms) continue;
     float *orow = p.out + (uint64_t)sbeam * p.out_beam_pitch + ((uint64_t)coarse * p.n_dms + dm) * p.out_dm_pitch;
 #pragma unroll
-    for (int k = 0; k < NP; ++k) {
-      const uint32_t t = t0 + 2 * lane + 128 * k;
-      const float a = (float)lo[j][k], b = (float)hi[j][k];
-      if (p.vec_store && t + 1 < p.n_samples) {
-        f32x2 v = {a, b};
-        if (p.nontemporal) __builtin_nontemporal_store(v, reinterpret_cast<f32x2 *>(orow + t));
-        else *reinterpret_cast<f32x2 *>(orow + t) = v;
+    for (int k = 0; k < NQ; ++k) {
+      const uint32_t t = t0 + 4 * lane + 256 * k;
+      const float v[4] = {(float)lo[j][2 * k], (float)hi[j][2 * k], (float)lo[j][2 * k + 1], (float)hi[j][2 * k + 1]};
+      if (p.vec_store && t + 3 < p.n_samples) {
+        f32x4 f = {v[0], v[1], v[2], v[3]};
+        if (p.nontemporal) __builtin_nontemporal_store(f, reinterpret_cast<f32x4 *>(orow + t));
+        else *reinterpret_cast<f32x4 *>(orow + t) = f;
       } else {
-        if (t < p.n_samples) orow[t] = a;
-        if (t + 1 < p.n_samples) orow[t + 1] = b;
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (t + e < p.n_samples) orow[t + e] = v[e];
       }
     }
   }
 }
 
+// Shapes with at most 32 sums per lane are held to four waves per SIMD (the
+// compiler would otherwise hoist every row's metadata reads at the cost of
+// a wave).
 template <int RUN, int DPW>
-__global__ void __launch_bounds__(512) rowsum_u16_staged(const RowSumF32Params p) {
-  extern __shared__ uint32_t lds[];
+constexpr int u16_min_waves() {
+  return RUN * DPW <= 32 ? 4 : 1;
+}
+
+template <int RUN, int DPW>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(u16_min_waves<RUN, DPW>())))
+rowsum_u16_staged(const RowSumF32Params p) {
+  extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
   u16_tile<RUN, DPW>(p, xcd_logical(blockIdx.x, p.n_tiles), lds);
 }
 
 template <int RUN, int DPW>
-__global__ void __launch_bounds__(512) rowsum_u16_batch(const RowSumF32Params *__restrict__ beams, const TileQueues q) {
-  extern __shared__ uint32_t lds[];
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(u16_min_waves<RUN, DPW>())))
+rowsum_u16_batch(const RowSumF32Params *__restrict__ beams, const TileQueues q) {
+  extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
   __shared__ uint32_t claimed;
   QueueCursor cur;
   for (;;) {
@@ -234,9 +261,9 @@ __global__ void __launch_bounds__(512) rowsum_u16_batch(const RowSumF32Params *_
 
 // One workgroup per listed (job, tile) descriptor, in XCD-aware order.
 template <int RUN, int DPW>
-__global__ void __launch_bounds__(512) rowsum_u16_list(const RowSumF32Params *__restrict__ jobs,
-                                                         const TileDesc *__restrict__ tiles, uint32_t n) {
-  extern __shared__ uint32_t lds[];
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(u16_min_waves<RUN, DPW>())))
+rowsum_u16_list(const RowSumF32Params *__restrict__ jobs, const TileDesc *__restrict__ tiles, uint32_t n) {
+  extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
   const TileDesc d = tiles[xcd_logical(blockIdx.x, n)];
   u16_tile<RUN, DPW>(jobs[d.job], d.tile, lds);
 }

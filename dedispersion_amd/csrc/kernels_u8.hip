@@ -85,7 +85,7 @@ template <int RUN, int DPW, bool LDS, bool FLUSH>
 __global__ void __launch_bounds__(512) rowsum_u8(const RowSumU8Params p) {
   constexpr int NG = RUN / 4;   // four-sample groups per lane
   constexpr int TS = 64 * RUN;  // samples per tile
-  extern __shared__ uint32_t lds[];
+  extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
 
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t wave = uniform(threadIdx.x >> 6);

@@ -39,7 +39,7 @@ __global__ void __launch_bounds__(512) rowsum_u8_factored(const RowSumU8Params p
   constexpr int TS = 64 * RUN;          // samples per tile
   constexpr uint32_t TD = kFactoredDMs; // DMs per tile
   constexpr uint32_t REC = kFactoredRecordDwords;
-  extern __shared__ uint32_t lds[];
+  extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
 
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t wave = uniform(threadIdx.x >> 6);

@@ -16,13 +16,16 @@
 //    once at the start.
 //  * Workgroups are ordered XCD-aware (staged_common.hpp).
 //
-// Packed rows (inputBits 1, 2 or 4, include/Dedispersion.hpp:92-103) are
-// loaded as packed bytes and unpacked to one byte per sample as they are
-// written to LDS; the summation then reads the same windows as for bytes.
+// Byte rows are staged in four copies, copy c shifted down by c bytes (built
+// in registers with v_alignbyte), so that every four-sample group of a DM is
+// one aligned ds_read_b32.  Packed rows (inputBits 1, 2 or 4,
+// include/Dedispersion.hpp:92-103) are loaded as packed bytes and unpacked
+// to one byte per sample into a single copy; their summation reads two
+// dwords per group and aligns them with v_perm.
 //
 // The summation is that of kernels_u8.hip: lane l owns the four-sample groups
-// t0 + 4l + 256k, reads the two dwords of a group with one ds_read2_b32 and
-// adds v_perm_b32-extracted 16-bit pairs; partial sums are exact integers.
+// t0 + 4l + 256k and adds v_perm_b32-extracted 16-bit pairs; partial sums
+// are exact integers.
 #include "staged_common.hpp"
 
 namespace ddamd {
@@ -32,6 +35,11 @@ using namespace staged;
 
 __device__ __forceinline__ uint32_t sel_even_of(uint32_t r) { return r | 0x0c00u | ((r + 2) << 16) | 0x0c000000u; }
 __device__ __forceinline__ uint32_t sel_odd_of(uint32_t r) { return (r + 1) | 0x0c00u | ((r + 3) << 16) | 0x0c000000u; }
+
+// One DM word of the row metadata, read alone: the relaxed atomic load keeps
+// the compiler from merging the words of several DMs into one broadcast
+// ds_read_b128, which serializes.
+__device__ __forceinline__ uint32_t dm_word(const uint32_t *p) { return __atomic_load_n(p, __ATOMIC_RELAXED); }
 
 // Unpacks 16 bytes of packed samples (log2 of samples per byte = lf, 1..3)
 // into one byte per sample at dst (16 << lf bytes).  Samples are the fields
@@ -115,11 +123,16 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
   const uint32_t NCH_IN = NCH >> lf;    // 16-byte chunks of packed input per window
   const uint32_t dmax = p.n_dms - 1;
 
-  // LDS: rows (packed row | beam << 16, float delay) | windows [2][G][WD] | shifts [2][G][TD]
+  // LDS: rows (packed row | beam << 16, float delay) | windows [2][G][COPIES][WD] | DM words [2][G][TD]
+  // Byte rows are kept in four copies, copy c shifted down by c bytes, so a
+  // DM's four-sample group at any offset is one aligned dword of copy
+  // (offset & 3); packed rows are unpacked into one copy.
+  constexpr uint32_t COPIES = PACKED ? 1 : 4;
+  const uint32_t win_off = (2 * p.live_pitch + 3) & ~3u;
   uint32_t *lrow = lds;
   float *ltab = reinterpret_cast<float *>(lds + p.live_pitch);
-  uint32_t *windows = lds + ((2 * p.live_pitch + 3) & ~3u);
-  uint32_t *meta = windows + 2 * G * WD;
+  uint32_t *windows = lds + win_off;
+  uint32_t *meta = windows + 2 * G * COPIES * WD;
 
   for (uint32_t k = threadIdx.x; k < nlive; k += blockDim.x) {
     const uint32_t row = live[k];
@@ -131,6 +144,7 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
 
   // ---- staging: wavefront w stages rows w and w + nwaves of a group ----
   u32x4 sv[SPW][CPL];
+  uint32_t sx[SPW][CPL];  // the dword after each chunk (byte rows)
   uint32_t so[SPW];
   auto stage_load = [&](uint32_t k0, uint32_t nn) {
 #pragma unroll
@@ -162,6 +176,7 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
 #pragma unroll
               for (int e = 0; e < 4; ++e) sv[s][c][e] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, 16 * m + 4 * e, 0, 0);
             }
+            if constexpr (!PACKED) sx[s][c] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, 16 * m + 16, 0, 0);
           }
         }
       }
@@ -172,21 +187,47 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
     for (int s = 0; s < SPW; ++s) {
       const uint32_t g = wave + s * nwaves;
       if (g < nn) {
-        uint32_t *w = windows + (buf * G + g) * WD;
-        if (!PACKED || lf == 0) {
+        uint32_t *w = windows + (buf * G + g) * COPIES * WD;
+        if constexpr (!PACKED) {
 #pragma unroll
           for (int c = 0; c < CPL; ++c) {
             const uint32_t m = lane + 64 * c;
-            if (m < NCH) *reinterpret_cast<u32x4 *>(w + 4 * m) = sv[s][c];
+            if (m < NCH) {
+              const u32x4 v = sv[s][c];
+              const uint32_t nx = sx[s][c];
+              u32x4 r1, r2, r3;
+#pragma unroll
+              for (int e = 0; e < 4; ++e) {
+                const uint32_t hi = e < 3 ? v[e + 1] : nx;
+                r1[e] = __builtin_amdgcn_alignbyte(hi, v[e], 1);
+                r2[e] = __builtin_amdgcn_alignbyte(hi, v[e], 2);
+                r3[e] = __builtin_amdgcn_alignbyte(hi, v[e], 3);
+              }
+              *reinterpret_cast<u32x4 *>(w + 4 * m) = v;
+              *reinterpret_cast<u32x4 *>(w + WD + 4 * m) = r1;
+              *reinterpret_cast<u32x4 *>(w + 2 * WD + 4 * m) = r2;
+              *reinterpret_cast<u32x4 *>(w + 3 * WD + 4 * m) = r3;
+            }
           }
+          // the DM's LDS byte address: copy (offset & 3), dword offset >> 2
+          const uint32_t o = so[s];
+          if (lane < TD) meta[(buf * G + g) * TD + lane] = 4 * (win_off + ((buf * G + g) * 4 + (o & 3)) * WD + (o >> 2));
         } else {
+          if (lf == 0) {
 #pragma unroll
-          for (int c = 0; c < CPL; ++c) {
-            const uint32_t m = lane + 64 * c;
-            if (m < NCH_IN) unpack_store(w + ((4 * m) << lf), sv[s][c], lf);
+            for (int c = 0; c < CPL; ++c) {
+              const uint32_t m = lane + 64 * c;
+              if (m < NCH) *reinterpret_cast<u32x4 *>(w + 4 * m) = sv[s][c];
+            }
+          } else {
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+              const uint32_t m = lane + 64 * c;
+              if (m < NCH_IN) unpack_store(w + ((4 * m) << lf), sv[s][c], lf);
+            }
           }
+          if (lane < TD) meta[(buf * G + g) * TD + lane] = so[s];
         }
-        if (lane < TD) meta[(buf * G + g) * TD + lane] = so[s];
       }
     }
   };
@@ -221,7 +262,48 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
 
   // Rows are taken two at a time so that the two extracted pairs of a DM
   // and sample group go into its accumulator with one three-input add.
+  // Byte rows: a DM's group is one aligned dword d of copy (offset & 3); its
+  // even and odd samples go to 16-bit pairs with constant v_perm selectors.
+  // The DM's LDS address is its metadata word (dm_word), read into a scalar
+  // register, plus the lane offset.
+  constexpr uint32_t SEL_EVEN = 0x0c020c00u, SEL_ODD = 0x0c030c01u;
+  const char *lane_base = reinterpret_cast<const char *>(lds) + 4 * lane;
+  auto compute_bytes = [&](uint32_t buf, uint32_t n) {
+    uint32_t g = 0;
+    for (; g + 1 < n; g += 2) {
+      const uint32_t *m0 = meta + (buf * G + g) * TD + dw;
+      const uint32_t *m1 = m0 + TD;
+#pragma unroll
+      for (int j = 0; j < DPW; ++j) {
+        const uint32_t *src0 = reinterpret_cast<const uint32_t *>(lane_base + uniform(dm_word(m0 + j)));
+        const uint32_t *src1 = reinterpret_cast<const uint32_t *>(lane_base + uniform(dm_word(m1 + j)));
+#pragma unroll
+        for (int k = 0; k < NG; ++k) {
+          const uint32_t x0 = src0[64 * k], x1 = src1[64 * k];
+          even[j][k] += __builtin_amdgcn_perm(x0, x0, SEL_EVEN) + __builtin_amdgcn_perm(x1, x1, SEL_EVEN);
+          odd[j][k] += __builtin_amdgcn_perm(x0, x0, SEL_ODD) + __builtin_amdgcn_perm(x1, x1, SEL_ODD);
+        }
+      }
+    }
+    if (g < n) {
+      const uint32_t *mrow = meta + (buf * G + g) * TD + dw;
+#pragma unroll
+      for (int j = 0; j < DPW; ++j) {
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(lane_base + uniform(dm_word(mrow + j)));
+#pragma unroll
+        for (int k = 0; k < NG; ++k) {
+          const uint32_t x = src[64 * k];
+          even[j][k] += __builtin_amdgcn_perm(x, x, SEL_EVEN);
+          odd[j][k] += __builtin_amdgcn_perm(x, x, SEL_ODD);
+        }
+      }
+    }
+  };
   auto compute = [&](uint32_t buf, uint32_t n) {
+    if constexpr (!PACKED) {
+      compute_bytes(buf, n);
+      return;
+    }
     uint32_t g = 0;
     for (; g + 1 < n; g += 2) {
       const uint32_t *m0 = meta + (buf * G + g) * TD + dw;
@@ -356,16 +438,24 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
 
 template <int RUN, int DPW, bool FLUSH, bool PACKED>
 __global__ void __launch_bounds__(512) rowsum_u8_staged(const RowSumU8Params p) {
-  extern __shared__ uint32_t lds[];
+  extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
   u8_tile<RUN, DPW, FLUSH, PACKED>(p, xcd_logical(blockIdx.x, p.n_tiles), lds);
 }
 
 // Persistent form: workgroups take (beam, tile) descriptors from XCD-local
-// work queues until every queue is empty (see staged_common.hpp).
+// work queues until every queue is empty (see staged_common.hpp).  Shapes
+// whose tile loop fits in 128 VGPRs (the per-tile kernels of the same shape
+// do) are held to four waves per SIMD: the queue loop around the tile can
+// otherwise push the allocation just past 128 and cost a wave.
+template <int RUN, int DPW, bool FLUSH>
+constexpr int batch_min_waves() {
+  return (!FLUSH && RUN * DPW <= 128) ? 4 : 1;
+}
+
 template <int RUN, int DPW, bool FLUSH, bool PACKED>
-__global__ void __launch_bounds__(512) rowsum_u8_batch(const RowSumU8Params *__restrict__ beams,
-                                                         const TileQueues q) {
-  extern __shared__ uint32_t lds[];
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(batch_min_waves<RUN, DPW, FLUSH>())))
+rowsum_u8_batch(const RowSumU8Params *__restrict__ beams, const TileQueues q) {
+  extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
   __shared__ uint32_t claimed;
   QueueCursor cur;
   for (;;) {
@@ -380,7 +470,7 @@ __global__ void __launch_bounds__(512) rowsum_u8_batch(const RowSumU8Params *__r
 template <int RUN, int DPW, bool FLUSH, bool PACKED>
 __global__ void __launch_bounds__(512) rowsum_u8_list(const RowSumU8Params *__restrict__ jobs,
                                                         const TileDesc *__restrict__ tiles, uint32_t n) {
-  extern __shared__ uint32_t lds[];
+  extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
   const TileDesc d = tiles[xcd_logical(blockIdx.x, n)];
   u8_tile<RUN, DPW, FLUSH, PACKED>(jobs[d.job], d.tile, lds);
 }

@@ -334,9 +334,19 @@ dd_status geometry_of(const dd_plan *p, dd_mode mode, const dd_conf &c, Geometry
       g.lds_bytes = c.unroll * g.win;
       if (stageable && g.win <= 16 * 64 * kU8StagedChunks) {
         const uint32_t live_pitch = mode == DD_SINGLE_STEP ? p->obs.nr_channels : cps(p);
+        // rows | [2 buffers][unroll rows][copies][win / 4 dwords] | DM words; byte
+        // rows are kept in four shifted copies, packed rows in one; rows
+        // staged per barrier are capped at what fits in LDS
+        const uint32_t copies = per > 1 ? 1 : 4;
+        auto lds_of = [&](uint32_t u) {
+          return 4 * (roundup(2 * live_pitch, 4) + 2 * u * copies * (g.win / 4) + 2 * u * tile);
+        };
+        uint32_t unroll = c.unroll;
+        while (unroll > 1 && lds_of(unroll) > p->max_lds) --unroll;
         g.staged = 1;
         g.packed = per > 1;
-        g.lds_bytes = 4 * (roundup(2 * live_pitch, 4) + 2 * c.unroll * (g.win / 4) + 2 * c.unroll * tile);
+        g.unroll = unroll;
+        g.lds_bytes = lds_of(unroll);
       } else if (per > 1) {
         return fail(DD_ERR_CONFIGURATION, "packed input needs the pipelined kernel, whose row window is at most " +
                                               std::to_string(16 * 64 * kU8StagedChunks) + " bytes; this one needs " +
@@ -520,9 +530,16 @@ dd_status prepare_step_two_u16(dd_plan *plan, const dd_conf &c, const void *d_in
     return fail(DD_ERR_CONFIGURATION, "step-two shift spread too large for the uint16 window");
   L.g.win = win;
   L.g.staged = 1;
-  L.g.lds_bytes = 4 * (roundup(2 * plan->obs.nr_subbands, 4) + 2 * c.unroll * win + 2 * c.unroll * tile);
+  // rows | [2 buffers][unroll rows][4 copies][win / 2 dwords] | DM addresses;
+  // rows staged per barrier are capped at what fits in LDS
+  auto lds_of = [&](uint32_t u) { return 4 * (roundup(2 * plan->obs.nr_subbands, 4) + 4 * u * win + 2 * u * tile); };
+  uint32_t unroll = c.unroll;
+  while (unroll > 1 && lds_of(unroll) > plan->max_lds) --unroll;
+  L.g.lds_bytes = lds_of(unroll);
   if (L.g.lds_bytes > plan->max_lds)
     return fail(DD_ERR_CONFIGURATION, "LDS demand " + std::to_string(L.g.lds_bytes) + " bytes exceeds the device limit");
+  L.g.unroll = unroll;
+  L.k.unroll = unroll;
   L.k.win_floats = win;
   L.k.combine_rows = 65535u / bound >= 4 ? 4 : 65535u / bound >= 2 ? 2 : 1;
   return DD_OK;

@@ -334,7 +334,7 @@ def pipeline_candidates(recs: List[Dict], top: int, small_lds: int) -> List[Dict
 
 
 def tune_batch(path: Optional[str], iterations: int = 3, top: int = 6, small_lds: int = 4,
-               chunk_counts=(2, 3, 4, 6)) -> Dict:
+               chunk_counts=(2, 3, 4, 6), reserves=(0, 24576, 32768, 40960, 53248, 65536, 81920)) -> Dict:
     """Tunes the 40-beam batch: an exhaustive sweep of the staged
     configurations of each kernel kind alone (persistent launches), then the
     pipelined graph over pairs of the best candidates of each kind and chunk
@@ -374,6 +374,27 @@ def tune_batch(path: Optional[str], iterations: int = 3, top: int = 6, small_lds
         pipe["swept"] = n
         print(f"pipelined: {n} pairs x chunk counts in {time.time() - t0:.1f}s; best {pipe}", file=sys.stderr,
               flush=True)
+        # LDS reserves: fewer workgroups of one kind per compute unit leave
+        # room for the other kind's beside them
+        t0 = time.time()
+        bs.set_conf(DD_STEP_ONE, Conf(**pipe["step_one"]))
+        bs.set_conf(DD_STEP_TWO, Conf(**pipe["step_two"]))
+        pipe["lds_reserve"] = {"step_one": 0, "step_two": 0}
+        n = 0
+        for r1 in reserves:
+            for r2 in reserves:
+                bs.batch.set_lds_reserve(DD_STEP_ONE, r1)
+                bs.batch.set_lds_reserve(DD_STEP_TWO, r2)
+                for chunks in chunk_counts:
+                    bs.batch.set_pipeline(chunks)
+                    try:
+                        ms = time_pipelined(bs, iterations + 2)
+                    except DDError:
+                        continue
+                    n += 1
+                    if ms < pipe["ms"]:
+                        pipe.update(ms=ms, chunks=chunks, lds_reserve={"step_one": r1, "step_two": r2})
+        print(f"LDS reserves: {n} settings in {time.time() - t0:.1f}s; best {pipe}", file=sys.stderr, flush=True)
     finally:
         bs.close()
     out = {"workload": "batched-beams-40x1536ch-two-step", "device": "MI355X", "confs": best, "pipeline": pipe}
@@ -384,8 +405,8 @@ def tune_batch(path: Optional[str], iterations: int = 3, top: int = 6, small_lds
 
 
 def load_batch_pipeline(path: str = BATCH_TUNED_PATH) -> Optional[Dict]:
-    """The tuned pipelined setting: {"confs": {mode: Conf}, "chunks": n}, or
-    None when the file holds none."""
+    """The tuned pipelined setting: {"confs": {mode: Conf}, "chunks": n,
+    "lds_reserve": {mode: bytes}}, or None when the file holds none."""
     try:
         with open(path) as f:
             p = json.load(f).get("pipeline")
@@ -393,8 +414,10 @@ def load_batch_pipeline(path: str = BATCH_TUNED_PATH) -> Optional[Dict]:
         return None
     if not p:
         return None
+    r = p.get("lds_reserve", {})
     return {"confs": {DD_STEP_ONE: Conf(**p["step_one"]), DD_STEP_TWO: Conf(**p["step_two"])},
-            "chunks": int(p["chunks"])}
+            "chunks": int(p["chunks"]),
+            "lds_reserve": {DD_STEP_ONE: int(r.get("step_one", 0)), DD_STEP_TWO: int(r.get("step_two", 0))}}
 
 
 def learned_tune(bench: Bench, model, budget: float, iterations: int) -> Dict:

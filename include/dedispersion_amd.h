@@ -272,6 +272,13 @@ dd_status dd_batch_set_compact_intermediate(dd_batch *batch, int enable);
  * joins back into the caller's stream (and its graph holds the whole DAG).
  * dd_batch_run_mode keeps the persistent per-kind launches. */
 dd_status dd_batch_set_pipeline(dd_batch *batch, uint32_t chunks);
+/* In the pipelined mode, each workgroup of kernel kind `mode` requests at
+ * least `bytes` of LDS (0, the default: what its kernel needs).  This caps
+ * how many workgroups of the kind share a compute unit, so that workgroups
+ * of the other kind fit beside them.  The persistent launches are not
+ * affected.  A reserve above the device limit fails with
+ * DD_ERR_CONFIGURATION. */
+dd_status dd_batch_set_lds_reserve(dd_batch *batch, dd_mode mode, uint32_t bytes);
 dd_status dd_batch_get_conf(const dd_batch *batch, dd_mode mode, dd_conf *conf);
 /* Builds the work queues (first run, or after a change) and enqueues the
  * batch on `stream`.  use_graph != 0: the launches are captured once into a

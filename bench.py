@@ -294,6 +294,8 @@ def run_batched(args, torch):
                    "tiles": info["tiles"], "workgroups": info["workgroups"],
                    "conf": confs},
         "plain_per_beam": {"value": units / (ms_plain / 1e3), "ms_per_step": ms_plain,
+                           "launch": "one launch per beam and step, each plan configured from the tuned files "
+                                     "by its DM counts, float intermediate",
                            "launches_per_step": 2 * len(ws)},
         "batched_vs_plain": ms_plain / ms,
         "intermediate": "uint16" if info["compact_intermediate"] else "float32",

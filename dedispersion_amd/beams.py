@@ -17,13 +17,16 @@ from .workloads import PADDING, Workload
 
 class BeamSet:
     def __init__(self, torch, workloads: List[Workload], padding: int = PADDING, confs: Optional[dict] = None,
-                 pulses: bool = True, input_bits: int = 8):
+                 pulses: bool = True, input_bits: int = 8, tuned_plans: bool = True):
         """confs: {mode: Conf} for the batch; None = the tuned batch
         configurations packaged with the library (the tuned pipelined pair
         and chunk count when there is one, else the best of each kind, else
         the defaults).
         input_bits 1, 2 or 4: packed input rows of uniform random fields
-        (the workloads' noise model and pulses apply to 8-bit input only)."""
+        (the workloads' noise model and pulses apply to 8-bit input only).
+        tuned_plans: each plan takes its step-one and step-two configurations
+        from the packaged tuned-configuration files by its DM counts (the
+        reference's lookup), which the one-launch-per-beam path runs."""
         self.torch = torch
         self.workloads = workloads
         self.padding = padding
@@ -31,11 +34,22 @@ class BeamSet:
         self.inputs, self.mids, self.outs = [], [], []
         s = torch.cuda.current_stream().cuda_stream
         self.stream = s
+        tuned = {}
+        if tuned_plans:
+            from .tuner import read_tuned_file, tuned_file_path
+
+            tuned = {name: read_tuned_file(tuned_file_path(name)) for name in ("step_one", "step_two")}
         for w in workloads:
             o = w.obs
             if o.nr_dms_subbanding == 0:
                 raise ValueError(f"{w.name}: a beam set runs the two-step path")
             plan = Plan(o, padding, input_bits=input_bits)
+            if tuned_plans:
+                for mode, name in ((DD_STEP_ONE, "step_one"), (DD_STEP_TWO, "step_two")):
+                    try:
+                        plan.apply_tuned(mode, tuned[name])
+                    except KeyError:
+                        pass
             flags = [0] * plan.pad(o.nr_channels, 4)
             for c in w.zapped:
                 flags[c] = 1

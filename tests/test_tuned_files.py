@@ -13,7 +13,7 @@ MODES = ("single_step", "step_one", "step_two")
 
 
 def test_packaged_files_follow_from_the_sweep_records():
-    lines = tuner.tuned_lines_from_records(learned.load_records())
+    lines = tuner.tuned_lines_from_records(learned.load_records(tuner.PLAN_RECORDS_PATH))
     assert sorted(lines) == sorted(MODES)
     for mode in MODES:
         with open(tuner.tuned_file_path(mode)) as f:

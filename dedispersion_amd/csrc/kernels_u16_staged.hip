@@ -49,6 +49,8 @@ __device__ __forceinline__ void u16_tile(const RowSumF32Params &p, uint32_t logi
   const uint32_t t0 = tile.x * TS;
   const uint32_t d0 = tile.y * TD;
   const uint32_t dw = wave * DPW;
+  // a wavefront whose DMs all lie past the last DM (a partial DM tile) only stages rows
+  const bool busy = d0 + dw < p.n_dms;
   const uint32_t coarse = tile.z % p.n_coarse;
   const uint32_t sbeam = tile.z / p.n_coarse;
   const uint32_t G = p.unroll;
@@ -202,7 +204,7 @@ __device__ __forceinline__ void u16_tile(const RowSumF32Params &p, uint32_t logi
     const uint32_t sn = s0 + G;
     const uint32_t nn = sn < p.n_rows ? min(G, p.n_rows - sn) : 0;
     if (nn) stage_load(sn, nn);
-    compute(cur, n);
+    if (busy) compute(cur, n);
     if (nn) stage_store(cur ^ 1, nn);
     __syncthreads();
     cur ^= 1;

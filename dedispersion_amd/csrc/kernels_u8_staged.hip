@@ -112,6 +112,8 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
   const uint32_t t0 = tile.x * TS;
   const uint32_t d0 = tile.y * TD;
   const uint32_t dw = wave * DPW;  // first DM of this wavefront, relative to d0
+  // a wavefront whose DMs all lie past the last DM (a partial DM tile) only stages rows
+  const bool busy = d0 + dw < p.n_dms;
   const uint32_t group = tile.z % p.n_groups;
   const uint32_t beam = tile.z / p.n_groups;
   const uint32_t nlive = p.live_count[group];
@@ -364,7 +366,7 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
       }
       since_flush += n;
     }
-    compute(cur, n);
+    if (busy) compute(cur, n);
     if (nn) stage_store(cur ^ 1, nn);
     __syncthreads();
     cur ^= 1;

@@ -142,30 +142,31 @@ dd_status rebuild_live(dd_plan *p) {
 
 // Defaults: the fastest configurations of an exhaustive sweep on MI355X for
 // the 1536-channel, 24576-sample single-step and two-step workloads.
-// Defaults: the fastest configurations of exhaustive sweeps on MI355X -- the
-// ARTS single step (1536 channels, 2048 DMs), step one at 32 and 64 coarse
-// DMs and step two at 32 fine DMs over the ARTS-channel layouts.
+// Defaults: the fastest configurations of exhaustive sweeps on MI355X of the
+// ARTS layouts -- the single step (1536 channels, 2048 DMs, 24576 samples)
+// and the two-step path (32 subbands, 64 coarse x 32 fine DMs).
 dd_conf default_conf(dd_mode mode) {
   dd_conf c{};
   c.threads_d0 = 64;
   c.split_batches = 0;
   c.local = 1;
-  c.threads_d1 = 8;
   switch (mode) {
   case DD_SINGLE_STEP:
+    c.threads_d1 = 8;
     c.items_d0 = 16;
     c.items_d1 = 8;
     c.unroll = 4;
     break;
   case DD_STEP_ONE:
+    c.threads_d1 = 4;
     c.items_d0 = 8;
-    c.items_d1 = 4;
-    c.unroll = 8;
+    c.items_d1 = 8;
+    c.unroll = 4;
     break;
   default:
-    c.local = 2;
+    c.threads_d1 = 8;
     c.items_d0 = 8;
-    c.items_d1 = 2;
+    c.items_d1 = 4;
     c.unroll = 16;
     break;
   }

@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="batched-beams",
-                    choices=["batched-beams", "arts-two-step", "arts-single"])
+                    choices=["batched-beams", "arts-two-step", "arts-single", "detect-peaks"])
     ap.add_argument("--input-bits", type=int, default=8, choices=[1, 2, 4, 8],
                     help="bits per input sample (1, 2, 4 = packed rows of uniform random fields)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -313,6 +313,65 @@ def run_batched(args, torch):
     return line, ws[0], "two-step"
 
 
+def run_detect(args, torch):
+    """The step after the path: the boxcar peak search (widths 1..64) over
+    every dedispersed row of the 40-beam batch's outputs, one launch per
+    beam; the batch runs once beforehand to fill the outputs."""
+    from dedispersion_amd import PEAK_DTYPE, detect_peaks, workloads
+    from dedispersion_amd.beams import BeamSet
+
+    ws = workloads.batched_beams()
+    bs = BeamSet(torch, ws, input_bits=args.input_bits)
+    bs.run(use_graph=False)
+    s = torch.cuda.current_stream().cuda_stream
+    jobs = []
+    for plan, out in zip(bs.plans, bs.outs):
+        o, z = plan.obs, plan.sizes
+        rows = o.nr_synthesized_beams * o.nr_dms_subbanding * o.nr_dms
+        peaks = torch.empty(rows * PEAK_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+        jobs.append((out.data_ptr(), rows, z.nr_samples, z.out_pitch, peaks))
+    units = sum(r * n for _, r, n, _, _ in jobs)
+    nbytes = sum(r * (4 * n + PEAK_DTYPE.itemsize) for _, r, n, _, _ in jobs)
+
+    def step():
+        for ptr, r, n, pitch, peaks in jobs:
+            detect_peaks(ptr, r, n, pitch, 7, peaks.data_ptr(), s)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    start, stop = events(torch, 2)
+    t0 = time.perf_counter()
+    start.record()
+    for _ in range(args.steps):
+        step()
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = start.elapsed_time(stop) / args.steps
+    achieved = nbytes / (ms / 1e3)
+    line = {
+        "metric": "DM-trial*samples/s",
+        "value": units / (ms / 1e3),
+        "unit": "DM-trial*samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "i32/i64/f64",
+        "data": "the dedispersed outputs of the 40-beam batch (synthetic input)",
+        "config": {"workload": "detect-peaks-40-beams", "rows": sum(r for _, r, _, _, _ in jobs),
+                   "widths": [1 << j for j in range(7)], "launches_per_step": len(jobs)},
+        "roofline": {"bound": "hbm", "kernel": "detect_peaks", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None, "algorithmic_bytes": nbytes},
+        "wall_s": wall,
+    }
+    return line, None, "detect"
+
+
 def run_single_plan(args, torch):
     from dedispersion_amd import workloads
 
@@ -388,9 +447,11 @@ def main():
         return
     if args.workload == "batched-beams":
         line, w, mode = run_batched(args, torch)
+    elif args.workload == "detect-peaks":
+        line, w, mode = run_detect(args, torch)
     else:
         line, w, mode = run_single_plan(args, torch)
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and w is not None:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(w, mode, args.cpu_seconds, args.input_bits)
     print(json.dumps(line), flush=True)

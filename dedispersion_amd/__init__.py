@@ -30,6 +30,8 @@ __all__ = [
     "load",
     "synth_gaussian_u8",
     "synth_uniform_u8",
+    "detect_peaks",
+    "PEAK_DTYPE",
 ]
 
 
@@ -310,3 +312,16 @@ def synth_gaussian_u8(d_data: int, rows: int, cols: int, pitch: int, mean: float
 
 def synth_uniform_u8(d_data: int, rows: int, cols: int, pitch: int, seed: int, stream: int = 0) -> None:
     check(load().dd_synth_uniform_u8(d_data, rows, cols, pitch, seed, stream or None))
+
+
+# One dd_peak as a numpy record (copy a device array of them to the host with
+# this dtype).
+PEAK_DTYPE = np.dtype([("snr", "<f4"), ("mean", "<f4"), ("sigma", "<f4"), ("sample", "<u4"), ("width", "<u4")])
+
+
+def detect_peaks(d_rows: int, nr_rows: int, nr_samples: int, pitch: int, nr_widths: int, d_peaks: int,
+                 stream: int = 0) -> None:
+    """Boxcar peak search (dd_detect_peaks) over nr_rows device rows of
+    nr_samples integer-valued floats, pitch floats apart; d_peaks receives
+    nr_rows dd_peak records (PEAK_DTYPE)."""
+    check(load().dd_detect_peaks(d_rows, nr_rows, nr_samples, pitch, nr_widths, d_peaks, stream or None))

@@ -87,6 +87,16 @@ class DDSizes(ctypes.Structure):
     ]
 
 
+class DDPeak(ctypes.Structure):
+    _fields_ = [
+        ("snr", c_float),
+        ("mean", c_float),
+        ("sigma", c_float),
+        ("sample", c_uint32),
+        ("width", c_uint32),
+    ]
+
+
 class DDBatchInfo(ctypes.Structure):
     _fields_ = [
         ("jobs", c_uint32),
@@ -146,6 +156,7 @@ PROTOTYPES = {
     "dd_dedisperse": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_void_p]),
     "dd_subband_step_one": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "dd_subband_step_two": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_void_p]),
+    "dd_detect_peaks": (c_int, [c_void_p, c_uint64, c_uint32, c_uint64, c_uint32, c_void_p, c_void_p]),
     "dd_tune_time": (
         c_int,
         [c_void_p, c_int, POINTER(DDConf), c_void_p, c_void_p, c_uint32, c_uint32, c_void_p, POINTER(c_float),

@@ -205,6 +205,31 @@ dd_status dd_subband_step_two(dd_plan *plan, const float *d_intermediate, float 
                               uint32_t first_synthesized_beam, uint32_t nr_synthesized_beams,
                               void *stream);
 
+/* ---- the step after: boxcar peak search over dedispersed rows ----
+ *
+ * No counterpart in the reference, whose output is the dedispersed array;
+ * this is the search a pipeline runs next.  For each of nr_rows rows of
+ * nr_samples integer-valued floats (the outputs above; a fractional part is
+ * truncated; values below 2^24, and nr_samples x max^2 below 2^63), pitch
+ * floats apart, it finds the boxcar of width w in {1, 2, 4, ...,
+ * 2^(nr_widths - 1)} (nr_widths <= 7) and start t (t + w <= nr_samples)
+ * with the highest
+ *     snr = (sum_{i<w} y(t + i) - w * mean) / (sigma * sqrt(w)),
+ * mean and sigma (population) being those of the row, computed in double
+ * from the exact integer sums.  Ties go to the smaller width, then the
+ * smaller start; snr is 0 for a constant row.  d_peaks has nr_rows entries
+ * (device memory). */
+typedef struct dd_peak {
+  float snr;
+  float mean;
+  float sigma;
+  uint32_t sample;  /* start t of the best window */
+  uint32_t width;   /* its width */
+} dd_peak;
+
+dd_status dd_detect_peaks(const float *d_rows, uint64_t nr_rows, uint32_t nr_samples, uint64_t pitch,
+                          uint32_t nr_widths, dd_peak *d_peaks, void *stream);
+
 /* ---- tuning harness (replaces src/DedispersionTuning.cpp:301-428) ---- */
 
 /* Times one mode under `conf` without changing the plan's configuration:

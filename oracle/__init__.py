@@ -81,6 +81,9 @@ def load() -> ctypes.CDLL:
     lib.orc_step_two_mt.restype = None
     lib.orc_step_two_mt.argtypes = [P(OrcObs), P(c_uint32), c_void_p, P(c_float), P(c_float), c_uint32,
                                     c_uint64, c_uint64, c_int]
+    lib.orc_detect_peaks.restype = None
+    lib.orc_detect_peaks.argtypes = [P(c_float), c_uint64, c_uint32, c_uint64, c_uint32, P(c_float), P(c_float),
+                                     P(c_float), P(c_uint32), P(c_uint32)]
     _lib = lib
     return lib
 
@@ -216,3 +219,16 @@ class Oracle:
         self.lib.orc_step_two_mt(ctypes.byref(o), _u32(m), inp.ctypes.data, _f32(out), _f32(s), self.padding,
                                  r0, r1, self.threads)
         return out.reshape(o.nr_synthesized_beams, o.nr_dms_sub * o.nr_dms, self.out_pitch)
+
+
+def detect_peaks(rows: np.ndarray, nr_samples: int, nr_widths: int) -> dict:
+    """orc_detect_peaks over a [rows][pitch] float array: per row snr, mean,
+    sigma, sample and width of the best boxcar."""
+    lib = load()
+    rows = np.ascontiguousarray(rows, dtype=np.float32)
+    n, pitch = rows.shape
+    out = {k: np.zeros(n, dtype=np.float32) for k in ("snr", "mean", "sigma")}
+    out.update({k: np.zeros(n, dtype=np.uint32) for k in ("sample", "width")})
+    lib.orc_detect_peaks(_f32(rows), n, nr_samples, pitch, nr_widths, _f32(out["snr"]), _f32(out["mean"]),
+                         _f32(out["sigma"]), _u32(out["sample"]), _u32(out["width"]))
+    return out

@@ -301,3 +301,55 @@ void orc_step_two_mt(const orc_obs *o, const uint32_t *beam_mapping,
   mt_job j = {2, o, NULL, beam_mapping, input, output, shifts, padding, row_begin, row_end};
   mt_dispatch(j, nthreads);
 }
+
+/* Boxcar peak search, straight loops: every window of every width is summed
+ * from the samples (no prefix sums), so a mistake in a running sum of the
+ * device kernel cannot be shared here. */
+void orc_detect_peaks(const float *rows_in, uint64_t rows, uint32_t samples, uint64_t pitch,
+                      uint32_t nr_widths, float *out_snr, float *out_mean, float *out_sigma,
+                      uint32_t *out_sample, uint32_t *out_width) {
+  for (uint64_t r = 0; r < rows; r++) {
+    const float *y = rows_in + r * pitch;
+    int64_t sum = 0, sq = 0;
+    for (uint32_t t = 0; t < samples; t++) {
+      const int64_t v = (int64_t)(int32_t)y[t];
+      sum += v;
+      sq += v * v;
+    }
+    const double dS = (double)samples;
+    const double mean = (double)sum / dS;
+    double var = (double)sq / dS - mean * mean;
+    if (!(var > 0.0)) var = 0.0;
+    const double sigma = sqrt(var);
+    double best = 0.0;
+    uint32_t best_t = 0, best_w = 0;
+    int found = 0;
+    for (uint32_t j = 0; j < nr_widths; j++) {
+      const uint32_t w = 1u << j;
+      if (w > samples) break;
+      int64_t bmax = -1;
+      uint32_t bt = 0;
+      for (uint32_t t = 0; t + w <= samples; t++) {
+        int64_t b = 0;
+        for (uint32_t i = 0; i < w; i++) b += (int64_t)(int32_t)y[t + i];
+        if (b > bmax) {
+          bmax = b;
+          bt = t;
+        }
+      }
+      const double width = (double)w;
+      const double snr = sigma > 0.0 ? ((double)bmax - width * mean) / (sigma * sqrt(width)) : 0.0;
+      if (!found || snr > best) {
+        found = 1;
+        best = snr;
+        best_t = bt;
+        best_w = w;
+      }
+    }
+    out_snr[r] = (float)best;
+    out_mean[r] = (float)mean;
+    out_sigma[r] = (float)sigma;
+    out_sample[r] = best_t;
+    out_width[r] = best_w;
+  }
+}

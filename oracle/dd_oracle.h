@@ -129,6 +129,16 @@ void orc_step_two_mt(const orc_obs *o, const uint32_t *beam_mapping,
                      uint32_t padding, uint64_t row_begin, uint64_t row_end,
                      int nthreads);
 
+/* The step after the path (no counterpart in the reference): for each of
+ * `rows` integer-valued rows (values truncated to integers), the boxcar of
+ * width 2^j (j < nr_widths) and start t (t + w <= samples) with the highest
+ * (B - w * mean) / (sigma * sqrt(w)), mean and sigma (population) from the
+ * exact integer sums; ties to the smaller width, then the smaller start.
+ * out_* have `rows` entries. */
+void orc_detect_peaks(const float *rows_in, uint64_t rows, uint32_t samples, uint64_t pitch,
+                      uint32_t nr_widths, float *out_snr, float *out_mean, float *out_sigma,
+                      uint32_t *out_sample, uint32_t *out_width);
+
 #ifdef __cplusplus
 }
 #endif

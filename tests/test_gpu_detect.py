@@ -1,0 +1,90 @@
+"""The step after the path: the device boxcar peak search (dd_detect_peaks)
+against the oracle's straight-loop restatement, and on single-step
+dedispersed output, where an injected pulse must be found at its DM and
+time."""
+import numpy as np
+import pytest
+
+import oracle
+from dedispersion_amd import PEAK_DTYPE, Plan, detect_peaks, workloads
+from tests.conftest import host_threads
+from tests.gpu_helpers import stream_of
+
+pytestmark = pytest.mark.gpu
+
+
+def run(torch, rows: np.ndarray, S: int, nw: int, offset: int = 0):
+    d_rows = torch.from_numpy(np.ascontiguousarray(rows).ravel()).cuda()
+    d_peaks = torch.zeros(rows.shape[0] * PEAK_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    detect_peaks(d_rows.data_ptr() + 4 * offset, rows.shape[0], S, rows.shape[1], nw, d_peaks.data_ptr(),
+                 stream_of(torch))
+    torch.cuda.synchronize()
+    return d_peaks.cpu().numpy().view(PEAK_DTYPE)
+
+
+def compare(got, want, what):
+    for k in ("sample", "width"):
+        bad = np.flatnonzero(got[k] != want[k])
+        assert bad.size == 0, f"{what}: {k} differs at rows {bad[:5]}: {got[k][bad[:5]]} vs {want[k][bad[:5]]}"
+    for k in ("snr", "mean", "sigma"):
+        np.testing.assert_allclose(got[k], want[k], rtol=1e-6, atol=1e-30, err_msg=f"{what} {k}")
+
+
+@pytest.mark.parametrize("S,pitch", [(1000, 1024), (20000, 20032), (8192 + 30, 8192 + 30), (32768, 32768), (5, 5)],
+                         ids=["1000", "3-segments", "unaligned-pitch", "32768", "tiny"])
+@pytest.mark.parametrize("nw", [1, 4, 7])
+def test_random_rows_against_oracle(torch_cuda, S, pitch, nw):
+    rng = np.random.default_rng(S + nw)
+    rows = rng.integers(0, 60000, size=(24, pitch)).astype(np.float32)
+    for r in range(0, 24, 3):  # pulses of widths 1..64 across segment borders
+        w = 1 << (r % 7)
+        t0 = (8192 - w // 2 + 37 * r) % max(1, S - w) if S > w else 0
+        rows[r, t0:t0 + w] += 40000
+    rows[23, :] = 123.0  # a constant row
+    got = run(torch_cuda, rows, S, nw)
+    want = oracle.detect_peaks(rows, S, nw)
+    compare(got, want, f"S={S} widths={nw}")
+
+
+@pytest.mark.parametrize("dm_index,t0,width", [(40, 1000, 1), (17, 2500, 6), (55, 3000, 16)])
+def test_pulse_found_in_dedispersed_output(torch_cuda, dm_index, t0, width):
+    """Single step, 256 channels, 64 DMs of 0.5, 4096 samples of Gaussian
+    noise with one dispersed boxcar pulse: its DM row's best window overlaps
+    the pulse; the search agrees with the oracle on the rows around it."""
+    from dedispersion_amd import DD_SINGLE_STEP, synth_gaussian_u8
+
+    torch = torch_cuda
+    w = workloads.tiny(channels=256, samples=4096, dms=64, step=0.5, min_freq=1300.0, bandwidth=0.5, seed=5)
+    plan = Plan(w.obs, 128)
+    z = plan.sizes
+    s = stream_of(torch)
+    d_in = torch.zeros(z.input_elements, dtype=torch.uint8, device="cuda")
+    synth_gaussian_u8(d_in.data_ptr(), 256, z.nr_dispersed_samples, z.in_pitch, 128.0, 16.0, 77, s)
+    dm_value = float(np.float32(dm_index) * np.float32(0.5))
+    plan.inject_pulse(d_in.data_ptr(), z.in_pitch, z.nr_dispersed_samples, dm_value, t0, width, 12, s)
+    d_out = torch.empty(z.output_elements, dtype=torch.float32, device="cuda")
+    plan.dedisperse(d_in.data_ptr(), d_out.data_ptr(), stream=s)
+    rows = w.obs.nr_dms
+    d_peaks = torch.zeros(rows * PEAK_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    detect_peaks(d_out.data_ptr(), rows, z.nr_samples, z.out_pitch, 7, d_peaks.data_ptr(), s)
+    torch.cuda.synchronize()
+    peaks = d_peaks.cpu().numpy().view(PEAK_DTYPE)
+    out = d_out.cpu().numpy().reshape(rows, z.out_pitch)
+    near = list(range(max(0, dm_index - 2), min(rows, dm_index + 3)))
+    compare(peaks[near], oracle.detect_peaks(out[near], z.nr_samples, 7), "dedispersed rows")
+    pk = peaks[dm_index]
+    start, found = int(pk["sample"]), int(pk["width"])
+    assert pk["snr"] > 10, pk
+    assert start < t0 + width and t0 < start + found, pk
+    assert np.argmax(peaks["snr"]) in near, peaks["snr"][near]
+    plan.close()
+
+
+def test_rejects_bad_arguments(torch_cuda):
+    from dedispersion_amd import DDError
+
+    x = torch_cuda.zeros(64, dtype=torch_cuda.float32, device="cuda")
+    with pytest.raises(DDError, match="nr_widths"):
+        detect_peaks(x.data_ptr(), 1, 64, 64, 8, x.data_ptr())
+    with pytest.raises(DDError, match="nr_samples"):
+        detect_peaks(x.data_ptr(), 1, 65, 64, 3, x.data_ptr())

@@ -159,26 +159,52 @@ __device__ __forceinline__ void u16_tile(const RowSumF32Params &p, uint32_t logi
 
   // R rows are first added as 16-bit pairs (R x the largest subband sum
   // stays below 2^16: p.combine_rows >= R), then each half goes to its
-  // 32-bit sum with one SDWA add.  A DM's address is its metadata word (a
-  // broadcast read) plus the lane offset, one add per row and DM; one
-  // ds_read_b64 fetches a quad.
+  // 32-bit sum with one SDWA add.  The DM words of the R rows reach the
+  // wavefront with one LDS read (lane l < R DPW reads the word of row
+  // l / DPW, DM l % DPW) and are taken from their lanes into scalar
+  // registers (v_readlane); a DM's address is its word plus the lane
+  // offset, and one ds_read_b64 fetches a quad.
   const char *lane_base = reinterpret_cast<const char *>(lds) + 8 * lane;
   auto rows = [&](auto R, uint32_t buf, uint32_t g) {
-    const uint32_t *m = meta + (buf * G + g) * TD + dw;  // broadcast reads
+    constexpr int NR = R.value;
+    const uint32_t words = meta[(buf * G + g + (lane / DPW) % NR) * TD + dw + lane % DPW];
+    // rows [0, PF) of DM j + 1 are loaded before DM j is summed, the others
+    // (R = 4) at DM j's own step
+    constexpr int PF = NR > 2 ? 2 : NR;
+    auto src_of = [&](int r, int j) {
+      return reinterpret_cast<const u32x2 *>(lane_base + (uint32_t)__builtin_amdgcn_readlane((int)words, r * DPW + j));
+    };
+    u32x2 xa[2][PF][NQ];  // [DM parity][row][quad]
+    auto load = [&](int j, u32x2 (&d)[PF][NQ]) {
+#pragma unroll
+      for (int r = 0; r < PF; ++r) {
+        const u32x2 *src = src_of(r, j);
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) d[r][k] = src[64 * k];
+      }
+    };
+    load(0, xa[0]);
 #pragma unroll
     for (int j = 0; j < DPW; ++j) {
-      const u32x2 *src[R.value];
+      u32x2 xb[NR - PF + 1][NQ];
 #pragma unroll
-      for (int r = 0; r < R.value; ++r) src[r] = reinterpret_cast<const u32x2 *>(lane_base + m[r * TD + j]);
+      for (int r = PF; r < NR; ++r) {
+        const u32x2 *src = src_of(r, j);
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) xb[r - PF][k] = src[64 * k];
+      }
+      if (j + 1 < DPW) load(j + 1, xa[(j + 1) & 1]);
 #pragma unroll
       for (int k = 0; k < NQ; ++k) {
-        u32x2 x = src[0][64 * k];
+        u32x2 v = xa[j & 1][0][k];
 #pragma unroll
-        for (int r = 1; r < R.value; ++r) x += src[r][64 * k];
-        lo[j][2 * k] += x[0] & 0xFFFFu;
-        hi[j][2 * k] += x[0] >> 16;
-        lo[j][2 * k + 1] += x[1] & 0xFFFFu;
-        hi[j][2 * k + 1] += x[1] >> 16;
+        for (int r = 1; r < PF; ++r) v += xa[j & 1][r][k];
+#pragma unroll
+        for (int r = PF; r < NR; ++r) v += xb[r - PF][k];
+        lo[j][2 * k] += v[0] & 0xFFFFu;
+        hi[j][2 * k] += v[0] >> 16;
+        lo[j][2 * k + 1] += v[1] & 0xFFFFu;
+        hi[j][2 * k + 1] += v[1] >> 16;
       }
     }
   };
@@ -232,23 +258,15 @@ __device__ __forceinline__ void u16_tile(const RowSumF32Params &p, uint32_t logi
   }
 }
 
-// Shapes with at most 32 sums per lane are held to four waves per SIMD (the
-// compiler would otherwise hoist every row's metadata reads at the cost of
-// a wave).
 template <int RUN, int DPW>
-constexpr int u16_min_waves() {
-  return RUN * DPW <= 32 ? 4 : 1;
-}
-
-template <int RUN, int DPW>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(u16_min_waves<RUN, DPW>())))
+__global__ void __launch_bounds__(512)
 rowsum_u16_staged(const RowSumF32Params p) {
   extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
   u16_tile<RUN, DPW>(p, xcd_logical(blockIdx.x, p.n_tiles), lds);
 }
 
 template <int RUN, int DPW>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(u16_min_waves<RUN, DPW>())))
+__global__ void __launch_bounds__(512)
 rowsum_u16_batch(const RowSumF32Params *__restrict__ beams, const TileQueues q) {
   extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
   __shared__ uint32_t claimed;
@@ -263,7 +281,7 @@ rowsum_u16_batch(const RowSumF32Params *__restrict__ beams, const TileQueues q) 
 
 // One workgroup per listed (job, tile) descriptor, in XCD-aware order.
 template <int RUN, int DPW>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(u16_min_waves<RUN, DPW>())))
+__global__ void __launch_bounds__(512)
 rowsum_u16_list(const RowSumF32Params *__restrict__ jobs, const TileDesc *__restrict__ tiles, uint32_t n) {
   extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
   const TileDesc d = tiles[xcd_logical(blockIdx.x, n)];

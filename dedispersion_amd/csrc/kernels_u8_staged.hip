@@ -36,11 +36,6 @@ using namespace staged;
 __device__ __forceinline__ uint32_t sel_even_of(uint32_t r) { return r | 0x0c00u | ((r + 2) << 16) | 0x0c000000u; }
 __device__ __forceinline__ uint32_t sel_odd_of(uint32_t r) { return (r + 1) | 0x0c00u | ((r + 3) << 16) | 0x0c000000u; }
 
-// One DM word of the row metadata, read alone: the relaxed atomic load keeps
-// the compiler from merging the words of several DMs into one broadcast
-// ds_read_b128, which serializes.
-__device__ __forceinline__ uint32_t dm_word(const uint32_t *p) { return __atomic_load_n(p, __ATOMIC_RELAXED); }
-
 // Unpacks 16 bytes of packed samples (log2 of samples per byte = lf, 1..3)
 // into one byte per sample at dst (16 << lf bytes).  Samples are the fields
 // of each byte from the least significant bit up, so dword e of the chunk
@@ -264,43 +259,67 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
 
   // Rows are taken two at a time so that the two extracted pairs of a DM
   // and sample group go into its accumulator with one three-input add.
-  // Byte rows: a DM's group is one aligned dword d of copy (offset & 3); its
-  // even and odd samples go to 16-bit pairs with constant v_perm selectors.
-  // The DM's LDS address is its metadata word (dm_word), read into a scalar
-  // register, plus the lane offset.
+  // The DM words of the two rows reach the wavefront with one LDS read:
+  // lane l < 2 DPW reads the word of row l / DPW, DM l % DPW, and each DM's
+  // word is then taken from its lane into a scalar register (v_readlane).
+  // The groups of DM j + 1 are loaded before DM j is summed.
+  // Byte rows: a DM's group is one aligned dword d of copy (offset & 3),
+  // addressed by its word plus the lane offset; its even and odd samples go
+  // to 16-bit pairs with constant v_perm selectors.
   constexpr uint32_t SEL_EVEN = 0x0c020c00u, SEL_ODD = 0x0c030c01u;
   const char *lane_base = reinterpret_cast<const char *>(lds) + 4 * lane;
+  const uint32_t lane_row = (lane / DPW) & 1, lane_dm = lane % DPW;
+  auto words_of = [&](uint32_t buf, uint32_t g, uint32_t rows) {
+    return meta[(buf * G + g + (rows > 1 ? lane_row : 0u)) * TD + dw + lane_dm];
+  };
+  auto word = [&](uint32_t words, int i) { return (uint32_t)__builtin_amdgcn_readlane((int)words, i); };
   auto compute_bytes = [&](uint32_t buf, uint32_t n) {
     uint32_t g = 0;
+    // the next pair's words are read one pair ahead (the last pair reads
+    // its own again)
+    uint32_t next = n > 1 ? words_of(buf, 0, 2) : 0u;
     for (; g + 1 < n; g += 2) {
-      const uint32_t *m0 = meta + (buf * G + g) * TD + dw;
-      const uint32_t *m1 = m0 + TD;
-#pragma unroll
-      for (int j = 0; j < DPW; ++j) {
-        const uint32_t *src0 = reinterpret_cast<const uint32_t *>(lane_base + uniform(dm_word(m0 + j)));
-        const uint32_t *src1 = reinterpret_cast<const uint32_t *>(lane_base + uniform(dm_word(m1 + j)));
+      const uint32_t words = next;
+      next = words_of(buf, min(g + 2, n - 2), 2);
+      uint32_t x[2][2][NG];  // [DM parity][row][group]
+      auto load = [&](int j, uint32_t (&x0)[NG], uint32_t (&x1)[NG]) {
+        const uint32_t *src0 = reinterpret_cast<const uint32_t *>(lane_base + word(words, j));
+        const uint32_t *src1 = reinterpret_cast<const uint32_t *>(lane_base + word(words, DPW + j));
 #pragma unroll
         for (int k = 0; k < NG; ++k) {
-          const uint32_t x0 = src0[64 * k], x1 = src1[64 * k];
-          even[j][k] += __builtin_amdgcn_perm(x0, x0, SEL_EVEN) + __builtin_amdgcn_perm(x1, x1, SEL_EVEN);
-          odd[j][k] += __builtin_amdgcn_perm(x0, x0, SEL_ODD) + __builtin_amdgcn_perm(x1, x1, SEL_ODD);
+          x0[k] = src0[64 * k];
+          x1[k] = src1[64 * k];
+        }
+      };
+      load(0, x[0][0], x[0][1]);
+#pragma unroll
+      for (int j = 0; j < DPW; ++j) {
+        if (j + 1 < DPW) load(j + 1, x[(j + 1) & 1][0], x[(j + 1) & 1][1]);
+        const uint32_t(&x0)[NG] = x[j & 1][0];
+        const uint32_t(&x1)[NG] = x[j & 1][1];
+#pragma unroll
+        for (int k = 0; k < NG; ++k) {
+          even[j][k] += __builtin_amdgcn_perm(x0[k], x0[k], SEL_EVEN) + __builtin_amdgcn_perm(x1[k], x1[k], SEL_EVEN);
+          odd[j][k] += __builtin_amdgcn_perm(x0[k], x0[k], SEL_ODD) + __builtin_amdgcn_perm(x1[k], x1[k], SEL_ODD);
         }
       }
     }
     if (g < n) {
-      const uint32_t *mrow = meta + (buf * G + g) * TD + dw;
+      const uint32_t words = words_of(buf, g, 1);
 #pragma unroll
       for (int j = 0; j < DPW; ++j) {
-        const uint32_t *src = reinterpret_cast<const uint32_t *>(lane_base + uniform(dm_word(mrow + j)));
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(lane_base + word(words, j));
 #pragma unroll
         for (int k = 0; k < NG; ++k) {
-          const uint32_t x = src[64 * k];
-          even[j][k] += __builtin_amdgcn_perm(x, x, SEL_EVEN);
-          odd[j][k] += __builtin_amdgcn_perm(x, x, SEL_ODD);
+          const uint32_t v = src[64 * k];
+          even[j][k] += __builtin_amdgcn_perm(v, v, SEL_EVEN);
+          odd[j][k] += __builtin_amdgcn_perm(v, v, SEL_ODD);
         }
       }
     }
   };
+  // Packed rows (one copy): a DM's group spans two dwords from its word's
+  // dword offset, aligned by the v_perm selectors of its byte offset.
   auto compute = [&](uint32_t buf, uint32_t n) {
     if constexpr (!PACKED) {
       compute_bytes(buf, n);
@@ -308,13 +327,12 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
     }
     uint32_t g = 0;
     for (; g + 1 < n; g += 2) {
-      const uint32_t *m0 = meta + (buf * G + g) * TD + dw;
-      const uint32_t *m1 = m0 + TD;
+      const uint32_t words = words_of(buf, g, 2);
       const uint32_t *win0 = windows + (buf * G + g) * WD + lane;
       const uint32_t *win1 = win0 + WD;
 #pragma unroll
       for (int j = 0; j < DPW; ++j) {
-        const uint32_t o0 = uniform(m0[j]), o1 = uniform(m1[j]);
+        const uint32_t o0 = word(words, j), o1 = word(words, DPW + j);
         const uint32_t *src0 = win0 + (o0 >> 2), *src1 = win1 + (o1 >> 2);
         const uint32_t se0 = sel_even_of(o0 & 3), sd0 = sel_odd_of(o0 & 3);
         const uint32_t se1 = sel_even_of(o1 & 3), sd1 = sel_odd_of(o1 & 3);
@@ -328,11 +346,11 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
       }
     }
     if (g < n) {
-      const uint32_t *mrow = meta + (buf * G + g) * TD + dw;
+      const uint32_t words = words_of(buf, g, 1);
       const uint32_t *win = windows + (buf * G + g) * WD + lane;
 #pragma unroll
       for (int j = 0; j < DPW; ++j) {
-        const uint32_t o = uniform(mrow[j]);
+        const uint32_t o = word(words, j);
         const uint32_t *src = win + (o >> 2);
         const uint32_t se = sel_even_of(o & 3), sd = sel_odd_of(o & 3);
 #pragma unroll
@@ -438,24 +456,27 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
   }
 }
 
+// Shapes whose sums fit in 64 VGPRs (RUN x DPW <= 128, no 32-bit flush
+// sums) are held to four waves per SIMD: the loads issued one DM ahead and
+// the queue loop of the persistent form can otherwise push the allocation
+// just past 128 and cost a wave.  The per-tile kernels of packed rows are
+// left to the compiler (held to 128 VGPRs, one of them spills).
+template <int RUN, int DPW, bool FLUSH, bool PACKED, bool PER_TILE>
+constexpr int u8_min_waves() {
+  return (!FLUSH && !(PACKED && PER_TILE) && RUN * DPW <= 128) ? 4 : 1;
+}
+
 template <int RUN, int DPW, bool FLUSH, bool PACKED>
-__global__ void __launch_bounds__(512) rowsum_u8_staged(const RowSumU8Params p) {
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(u8_min_waves<RUN, DPW, FLUSH, PACKED, true>())))
+rowsum_u8_staged(const RowSumU8Params p) {
   extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
   u8_tile<RUN, DPW, FLUSH, PACKED>(p, xcd_logical(blockIdx.x, p.n_tiles), lds);
 }
 
 // Persistent form: workgroups take (beam, tile) descriptors from XCD-local
-// work queues until every queue is empty (see staged_common.hpp).  Shapes
-// whose tile loop fits in 128 VGPRs (the per-tile kernels of the same shape
-// do) are held to four waves per SIMD: the queue loop around the tile can
-// otherwise push the allocation just past 128 and cost a wave.
-template <int RUN, int DPW, bool FLUSH>
-constexpr int batch_min_waves() {
-  return (!FLUSH && RUN * DPW <= 128) ? 4 : 1;
-}
-
+// work queues until every queue is empty (see staged_common.hpp).
 template <int RUN, int DPW, bool FLUSH, bool PACKED>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(batch_min_waves<RUN, DPW, FLUSH>())))
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(u8_min_waves<RUN, DPW, FLUSH, PACKED, false>())))
 rowsum_u8_batch(const RowSumU8Params *__restrict__ beams, const TileQueues q) {
   extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
   __shared__ uint32_t claimed;
@@ -470,8 +491,8 @@ rowsum_u8_batch(const RowSumU8Params *__restrict__ beams, const TileQueues q) {
 
 // One workgroup per listed (job, tile) descriptor, in XCD-aware order.
 template <int RUN, int DPW, bool FLUSH, bool PACKED>
-__global__ void __launch_bounds__(512) rowsum_u8_list(const RowSumU8Params *__restrict__ jobs,
-                                                        const TileDesc *__restrict__ tiles, uint32_t n) {
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(u8_min_waves<RUN, DPW, FLUSH, PACKED, false>())))
+rowsum_u8_list(const RowSumU8Params *__restrict__ jobs, const TileDesc *__restrict__ tiles, uint32_t n) {
   extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
   const TileDesc d = tiles[xcd_logical(blockIdx.x, n)];
   u8_tile<RUN, DPW, FLUSH, PACKED>(jobs[d.job], d.tile, lds);

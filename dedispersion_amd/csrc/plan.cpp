@@ -140,8 +140,6 @@ dd_status rebuild_live(dd_plan *p) {
   return DD_OK;
 }
 
-// Defaults: the fastest configurations of an exhaustive sweep on MI355X for
-// the 1536-channel, 24576-sample single-step and two-step workloads.
 // Defaults: the fastest configurations of exhaustive sweeps on MI355X of the
 // ARTS layouts -- the single step (1536 channels, 2048 DMs, 24576 samples)
 // and the two-step path (32 subbands, 64 coarse x 32 fine DMs).
@@ -159,7 +157,7 @@ dd_conf default_conf(dd_mode mode) {
     break;
   case DD_STEP_ONE:
     c.threads_d1 = 4;
-    c.items_d0 = 8;
+    c.items_d0 = 16;
     c.items_d1 = 8;
     c.unroll = 4;
     break;

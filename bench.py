@@ -149,6 +149,18 @@ def setup_single(torch, w, bits=8):
     return plan, kernels, units, (d_in, d_out)
 
 
+def repeat_for(fn, seconds: float, max_passes: int = 100):
+    """Runs fn until about `seconds` have passed (at least once); returns
+    (passes, elapsed seconds)."""
+    passes, dt = 0, 0.0
+    while passes < max_passes and (passes == 0 or dt < 0.9 * seconds):
+        t = time.perf_counter()
+        fn()
+        dt += time.perf_counter() - t
+        passes += 1
+    return passes, dt
+
+
 def cpu_baseline(w, mode: str, seconds: float, bits: int = 8):
     """The oracle on the host cores over a bounded sample of the workload."""
     import numpy as np
@@ -169,12 +181,15 @@ def cpu_baseline(w, mode: str, seconds: float, bits: int = 8):
         orc.step_two(mid.ravel(), rows=(0, o.nr_dms))
         per = time.perf_counter() - t
         n = max(1, min(o.nr_dms_subbanding, int(seconds / max(per, 1e-3))))
-        t = time.perf_counter()
-        mid = orc.step_one(inp, flags, rows=(0, n))
-        orc.step_two(mid.ravel(), rows=(0, n * o.nr_dms))
-        dt = time.perf_counter() - t
-        units = n * o.nr_dms * (o.nr_samples_per_batch // o.downsampling)
-        sample = f"{n} of {o.nr_dms_subbanding} coarse DMs (step one, all subbands, then step two, {o.nr_dms} fine DMs each) of {w.name}"
+
+        def one_pass():
+            mid = orc.step_one(inp, flags, rows=(0, n))
+            orc.step_two(mid.ravel(), rows=(0, n * o.nr_dms))
+
+        passes, dt = repeat_for(one_pass, seconds)
+        units = passes * n * o.nr_dms * (o.nr_samples_per_batch // o.downsampling)
+        sample = (f"{passes} pass(es) over {n} of {o.nr_dms_subbanding} coarse DMs (step one, all subbands, then "
+                  f"step two, {o.nr_dms} fine DMs each) of {w.name}")
     else:
         inp = rng.integers(0, 256, size=o.nr_beams * o.nr_channels * orc.in_pitch, dtype=np.uint8)
         flags = np.zeros(oracle.pad(o.nr_channels, 32), dtype=np.uint32)
@@ -183,11 +198,9 @@ def cpu_baseline(w, mode: str, seconds: float, bits: int = 8):
         orc.dedispersion(inp, flags, rows=(0, threads))
         per = (time.perf_counter() - t) / threads
         n = max(threads, min(o.nr_dms, int(seconds / max(per, 1e-4) * threads)))
-        t = time.perf_counter()
-        orc.dedispersion(inp, flags, rows=(0, n))
-        dt = time.perf_counter() - t
-        units = n * (o.nr_samples_per_batch // o.downsampling)
-        sample = f"{n} of {o.nr_dms} DM rows of {w.name}"
+        passes, dt = repeat_for(lambda: orc.dedispersion(inp, flags, rows=(0, n)), seconds)
+        units = passes * n * (o.nr_samples_per_batch // o.downsampling)
+        sample = f"{passes} pass(es) over {n} of {o.nr_dms} DM rows of {w.name}"
     return {"value": units / dt, "unit": "DM-trial*samples/s", "cores": threads, "kind": "port", "sample": sample,
             "seconds": round(dt, 3)}
 

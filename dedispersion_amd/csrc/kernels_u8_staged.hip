@@ -23,9 +23,11 @@
 // to one byte per sample into a single copy; their summation reads two
 // dwords per group and aligns them with v_perm.
 //
-// The summation is that of kernels_u8.hip: lane l owns the four-sample groups
-// t0 + 4l + 256k and adds v_perm_b32-extracted 16-bit pairs; partial sums
-// are exact integers.
+// The summation adds v_perm_b32-extracted 16-bit pairs, as kernels_u8.hip
+// does; partial sums are exact integers.  For byte rows each half-wavefront
+// sums one DM at a time (lane l: DM 2s + l / 32 of the wavefront's DMs, over
+// the groups (l % 32) + 32q), so every lane reads its own DM's LDS address;
+// packed rows keep lane l on the groups t0 + 4l + 256k of every DM.
 #include "staged_common.hpp"
 
 namespace ddamd {
@@ -111,9 +113,9 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
   const bool busy = d0 + dw < p.n_dms;
   const uint32_t group = tile.z % p.n_groups;
   const uint32_t beam = tile.z / p.n_groups;
-  const uint32_t nlive = p.live_count[group];
+  const uint32_t nlive = uniform(p.live_count[group]);
   const uint32_t *live = p.live + (uint64_t)group * p.live_pitch;
-  const uint32_t G = p.unroll;
+  const uint32_t G = uniform(p.unroll);
   const uint32_t WD = p.win_bytes / 4;  // dwords per window
   const uint32_t NCH = WD / 4;          // 16-byte chunks per window
   const uint32_t lf = PACKED ? p.in_log2f : 0u;  // log2 of samples per input byte
@@ -259,67 +261,84 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
 
   // Rows are taken two at a time so that the two extracted pairs of a DM
   // and sample group go into its accumulator with one three-input add.
-  // The DM words of the two rows reach the wavefront with one LDS read:
-  // lane l < 2 DPW reads the word of row l / DPW, DM l % DPW, and each DM's
-  // word is then taken from its lane into a scalar register (v_readlane).
-  // The groups of DM j + 1 are loaded before DM j is summed.
-  // Byte rows: a DM's group is one aligned dword d of copy (offset & 3),
-  // addressed by its word plus the lane offset; its even and odd samples go
-  // to 16-bit pairs with constant v_perm selectors.
+  //
+  // Byte rows: each half-wavefront takes one DM at a time.  With LPD = 32
+  // lanes per DM (64 when the wavefront has one DM), lane l sums, for DM
+  // slot s < SL, DM dw + s CPS + l / LPD over the four-sample groups
+  // (l % LPD) + LPD q, q < NGS: the 32 lanes of a half read 32 consecutive
+  // dwords (no bank conflicts), and each lane reads its DM's word itself
+  // (no v_readlane).  Chunk c (NG groups) of the lane is accumulator [c][k],
+  // slot c / CPS, groups (c % CPS) NG + k.  A group is one aligned dword of
+  // copy (offset & 3); even and odd samples go to 16-bit pairs with constant
+  // v_perm selectors.
   constexpr uint32_t SEL_EVEN = 0x0c020c00u, SEL_ODD = 0x0c030c01u;
-  const char *lane_base = reinterpret_cast<const char *>(lds) + 4 * lane;
-  const uint32_t lane_row = (lane / DPW) & 1, lane_dm = lane % DPW;
-  auto words_of = [&](uint32_t buf, uint32_t g, uint32_t rows) {
-    return meta[(buf * G + g + (rows > 1 ? lane_row : 0u)) * TD + dw + lane_dm];
-  };
-  auto word = [&](uint32_t words, int i) { return (uint32_t)__builtin_amdgcn_readlane((int)words, i); };
+  constexpr int LPD = DPW == 1 ? 64 : 32;  // lanes per DM
+  constexpr int CPS = 64 / LPD;            // DMs per slot = chunks per slot
+  constexpr int SL = DPW / CPS;            // DM slots per lane
+  const uint32_t lane_dm = lane / LPD;
+  const char *split_base = reinterpret_cast<const char *>(lds) + 4 * (lane % LPD);
+  const uint32_t *lane_meta = meta + dw + lane_dm;
+  auto word_of = [&](uint32_t buf, uint32_t g, int sl) { return lane_meta[uniform((buf * G + g) * TD) + sl * CPS]; };
   auto compute_bytes = [&](uint32_t buf, uint32_t n) {
     uint32_t g = 0;
-    // the next pair's words are read one pair ahead (the last pair reads
-    // its own again)
-    uint32_t next = n > 1 ? words_of(buf, 0, 2) : 0u;
     for (; g + 1 < n; g += 2) {
-      const uint32_t words = next;
-      next = words_of(buf, min(g + 2, n - 2), 2);
-      uint32_t x[2][2][NG];  // [DM parity][row][group]
-      auto load = [&](int j, uint32_t (&x0)[NG], uint32_t (&x1)[NG]) {
-        const uint32_t *src0 = reinterpret_cast<const uint32_t *>(lane_base + word(words, j));
-        const uint32_t *src1 = reinterpret_cast<const uint32_t *>(lane_base + word(words, DPW + j));
+      const uint32_t *src0[SL], *src1[SL];
+#pragma unroll
+      for (int sl = 0; sl < SL; ++sl) {
+        src0[sl] = reinterpret_cast<const uint32_t *>(split_base + word_of(buf, g, sl));
+        src1[sl] = reinterpret_cast<const uint32_t *>(split_base + word_of(buf, g + 1, sl));
+      }
+      // the groups of chunk c + 1 are loaded before chunk c is summed
+      uint32_t x[2][2][NG];  // [chunk parity][row][group]
+      auto load = [&](int c, uint32_t (&x0)[NG], uint32_t (&x1)[NG]) {
+        const int sl = c / CPS, q0 = (c % CPS) * NG;
 #pragma unroll
         for (int k = 0; k < NG; ++k) {
-          x0[k] = src0[64 * k];
-          x1[k] = src1[64 * k];
+          x0[k] = src0[sl][LPD * (q0 + k)];
+          x1[k] = src1[sl][LPD * (q0 + k)];
         }
       };
       load(0, x[0][0], x[0][1]);
 #pragma unroll
-      for (int j = 0; j < DPW; ++j) {
-        if (j + 1 < DPW) load(j + 1, x[(j + 1) & 1][0], x[(j + 1) & 1][1]);
-        const uint32_t(&x0)[NG] = x[j & 1][0];
-        const uint32_t(&x1)[NG] = x[j & 1][1];
+      for (int c = 0; c < DPW; ++c) {
+        if (c + 1 < DPW) load(c + 1, x[(c + 1) & 1][0], x[(c + 1) & 1][1]);
+        const uint32_t(&x0)[NG] = x[c & 1][0];
+        const uint32_t(&x1)[NG] = x[c & 1][1];
 #pragma unroll
         for (int k = 0; k < NG; ++k) {
-          even[j][k] += __builtin_amdgcn_perm(x0[k], x0[k], SEL_EVEN) + __builtin_amdgcn_perm(x1[k], x1[k], SEL_EVEN);
-          odd[j][k] += __builtin_amdgcn_perm(x0[k], x0[k], SEL_ODD) + __builtin_amdgcn_perm(x1[k], x1[k], SEL_ODD);
+          even[c][k] += __builtin_amdgcn_perm(x0[k], x0[k], SEL_EVEN) + __builtin_amdgcn_perm(x1[k], x1[k], SEL_EVEN);
+          odd[c][k] += __builtin_amdgcn_perm(x0[k], x0[k], SEL_ODD) + __builtin_amdgcn_perm(x1[k], x1[k], SEL_ODD);
         }
       }
     }
     if (g < n) {
-      const uint32_t words = words_of(buf, g, 1);
 #pragma unroll
-      for (int j = 0; j < DPW; ++j) {
-        const uint32_t *src = reinterpret_cast<const uint32_t *>(lane_base + word(words, j));
+      for (int sl = 0; sl < SL; ++sl) {
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(split_base + word_of(buf, g, sl));
 #pragma unroll
-        for (int k = 0; k < NG; ++k) {
-          const uint32_t v = src[64 * k];
-          even[j][k] += __builtin_amdgcn_perm(v, v, SEL_EVEN);
-          odd[j][k] += __builtin_amdgcn_perm(v, v, SEL_ODD);
+        for (int cc = 0; cc < CPS; ++cc) {
+          const int c = sl * CPS + cc;
+#pragma unroll
+          for (int k = 0; k < NG; ++k) {
+            const uint32_t v = src[LPD * (cc * NG + k)];
+            even[c][k] += __builtin_amdgcn_perm(v, v, SEL_EVEN);
+            odd[c][k] += __builtin_amdgcn_perm(v, v, SEL_ODD);
+          }
         }
       }
     }
   };
-  // Packed rows (one copy): a DM's group spans two dwords from its word's
-  // dword offset, aligned by the v_perm selectors of its byte offset.
+  // Packed rows (one copy) keep lane l on the groups t0 + 4l + 256k of every
+  // DM of the wavefront: the DM words of two rows reach the wavefront with
+  // one LDS read (lane l < 2 DPW reads the word of row l / DPW, DM l % DPW)
+  // and are taken from their lanes into scalar registers (v_readlane).
+  const uint32_t lane_row = (lane / DPW) & 1, lane_word = lane % DPW;
+  auto words_of = [&](uint32_t buf, uint32_t g, uint32_t rows) {
+    return meta[(buf * G + g + (rows > 1 ? lane_row : 0u)) * TD + dw + lane_word];
+  };
+  auto word = [&](uint32_t words, int i) { return (uint32_t)__builtin_amdgcn_readlane((int)words, i); };
+  // A packed row's group spans two dwords from its word's dword offset,
+  // aligned by the v_perm selectors of its byte offset.
   auto compute = [&](uint32_t buf, uint32_t n) {
     if constexpr (!PACKED) {
       compute_bytes(buf, n);
@@ -391,11 +410,16 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
   }
   flush();
 
+  // the DM and first sample of accumulator [j][k]
+  auto dm_of = [&](int j) -> uint32_t { return PACKED ? d0 + dw + j : d0 + dw + (j / CPS) * CPS + lane_dm; };
+  auto t_of = [&](int j, int k) -> uint32_t {
+    return PACKED ? t0 + 4 * lane + 256 * k : t0 + 4 * (lane % LPD) + 4 * LPD * ((j % CPS) * NG + k);
+  };
   if (p.out_u16) {
     // compact output: the four sums of a group as uint16 (each below 2^16)
 #pragma unroll
     for (int j = 0; j < DPW; ++j) {
-      const uint32_t dm = d0 + dw + j;
+      const uint32_t dm = dm_of(j);
       if (dm >= p.n_dms) continue;
       uint16_t *orow = reinterpret_cast<uint16_t *>(p.out) + (uint64_t)beam * p.out_beam_pitch +
                        (uint64_t)group * p.out_group_pitch + (uint64_t)dm * p.out_dm_pitch;
@@ -409,7 +433,7 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
           w0 = __builtin_amdgcn_perm(odd[j][k], even[j][k], 0x05040100u);
           w1 = __builtin_amdgcn_perm(odd[j][k], even[j][k], 0x07060302u);
         }
-        const uint32_t t = t0 + 4 * lane + 256 * k;
+        const uint32_t t = t_of(j, k);
         if (p.vec_store && t + 3 < p.n_samples) {
           u32x2 v = {w0, w1};
           *reinterpret_cast<u32x2 *>(orow + t) = v;
@@ -426,7 +450,7 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
 
 #pragma unroll
   for (int j = 0; j < DPW; ++j) {
-    const uint32_t dm = d0 + dw + j;
+    const uint32_t dm = dm_of(j);
     if (dm >= p.n_dms) continue;
     float *orow = p.out + (uint64_t)beam * p.out_beam_pitch + (uint64_t)group * p.out_group_pitch +
                   (uint64_t)dm * p.out_dm_pitch;
@@ -442,7 +466,7 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
         v[2] = (float)(even[j][k] >> 16);
         v[3] = (float)(odd[j][k] >> 16);
       }
-      const uint32_t t = t0 + 4 * lane + 256 * k;
+      const uint32_t t = t_of(j, k);
       if (p.vec_store && t + 3 < p.n_samples) {
         f32x4 f = {v[0], v[1], v[2], v[3]};
         if (p.nontemporal) __builtin_nontemporal_store(f, reinterpret_cast<f32x4 *>(orow + t));

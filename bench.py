@@ -162,47 +162,54 @@ def repeat_for(fn, seconds: float, max_passes: int = 100):
 
 
 def cpu_baseline(w, mode: str, seconds: float, bits: int = 8):
-    """The oracle on the host cores over a bounded sample of the workload."""
+    """The oracle on the host cores over a bounded sample of the workload,
+    and, beside it, on one thread (as the reference's CPU functions run)
+    over a shorter sample."""
     import numpy as np
 
     import oracle
 
-    threads = max(1, min(16, os.cpu_count() or 1))
-    orc = oracle.Oracle(w.obs, 128, threads=threads, input_bits=bits)
     o = w.obs
     rng = np.random.default_rng(0)
-    if mode == "two-step":
-        inp = rng.integers(0, 256, size=o.nr_beams * o.nr_channels * orc.sub_in_pitch, dtype=np.uint8)
-        flags = np.zeros(oracle.pad(o.nr_channels, 32), dtype=np.uint32)
-        flags[w.zapped] = 1
-        # calibrate on one coarse DM, then size the sample to ~seconds
-        t = time.perf_counter()
-        mid = orc.step_one(inp, flags, rows=(0, 1))
-        orc.step_two(mid.ravel(), rows=(0, o.nr_dms))
-        per = time.perf_counter() - t
-        n = max(1, min(o.nr_dms_subbanding, int(seconds / max(per, 1e-3))))
+    flags = np.zeros(oracle.pad(o.nr_channels, 32), dtype=np.uint32)
+    flags[w.zapped] = 1
 
-        def one_pass():
-            mid = orc.step_one(inp, flags, rows=(0, n))
-            orc.step_two(mid.ravel(), rows=(0, n * o.nr_dms))
+    def timed(threads: int, budget: float):
+        orc = oracle.Oracle(w.obs, 128, threads=threads, input_bits=bits)
+        if mode == "two-step":
+            inp = rng.integers(0, 256, size=o.nr_beams * o.nr_channels * orc.sub_in_pitch, dtype=np.uint8)
+            # calibrate on one coarse DM, then size the sample to ~budget
+            t = time.perf_counter()
+            mid = orc.step_one(inp, flags, rows=(0, 1))
+            orc.step_two(mid.ravel(), rows=(0, o.nr_dms))
+            per = time.perf_counter() - t
+            n = max(1, min(o.nr_dms_subbanding, int(budget / max(per, 1e-3))))
 
-        passes, dt = repeat_for(one_pass, seconds)
-        units = passes * n * o.nr_dms * (o.nr_samples_per_batch // o.downsampling)
-        sample = (f"{passes} pass(es) over {n} of {o.nr_dms_subbanding} coarse DMs (step one, all subbands, then "
-                  f"step two, {o.nr_dms} fine DMs each) of {w.name}")
-    else:
-        inp = rng.integers(0, 256, size=o.nr_beams * o.nr_channels * orc.in_pitch, dtype=np.uint8)
-        flags = np.zeros(oracle.pad(o.nr_channels, 32), dtype=np.uint32)
-        flags[w.zapped] = 1
-        t = time.perf_counter()
-        orc.dedispersion(inp, flags, rows=(0, threads))
-        per = (time.perf_counter() - t) / threads
-        n = max(threads, min(o.nr_dms, int(seconds / max(per, 1e-4) * threads)))
-        passes, dt = repeat_for(lambda: orc.dedispersion(inp, flags, rows=(0, n)), seconds)
-        units = passes * n * (o.nr_samples_per_batch // o.downsampling)
-        sample = f"{passes} pass(es) over {n} of {o.nr_dms} DM rows of {w.name}"
-    return {"value": units / dt, "unit": "DM-trial*samples/s", "cores": threads, "kind": "port", "sample": sample,
-            "seconds": round(dt, 3)}
+            def one_pass():
+                mid = orc.step_one(inp, flags, rows=(0, n))
+                orc.step_two(mid.ravel(), rows=(0, n * o.nr_dms))
+
+            passes, dt = repeat_for(one_pass, budget)
+            units = passes * n * o.nr_dms * (o.nr_samples_per_batch // o.downsampling)
+            sample = (f"{passes} pass(es) over {n} of {o.nr_dms_subbanding} coarse DMs (step one, all subbands, "
+                      f"then step two, {o.nr_dms} fine DMs each) of {w.name}")
+        else:
+            inp = rng.integers(0, 256, size=o.nr_beams * o.nr_channels * orc.in_pitch, dtype=np.uint8)
+            t = time.perf_counter()
+            orc.dedispersion(inp, flags, rows=(0, threads))
+            per = (time.perf_counter() - t) / threads
+            n = max(threads, min(o.nr_dms, int(budget / max(per, 1e-4) * threads)))
+            passes, dt = repeat_for(lambda: orc.dedispersion(inp, flags, rows=(0, n)), budget)
+            units = passes * n * (o.nr_samples_per_batch // o.downsampling)
+            sample = f"{passes} pass(es) over {n} of {o.nr_dms} DM rows of {w.name}"
+        return units / dt, sample, dt
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    value, sample, dt = timed(threads, seconds)
+    value1, sample1, dt1 = timed(1, seconds / 3)
+    return {"value": value, "unit": "DM-trial*samples/s", "cores": threads, "kind": "port", "sample": sample,
+            "seconds": round(dt, 3),
+            "single_thread": {"value": value1, "cores": 1, "sample": sample1, "seconds": round(dt1, 3)}}
 
 
 def measured_traffic(workload: str, kernel: str, conf: str):

@@ -23,9 +23,10 @@
 //    sums stay below 2^16), then each half of the result is added to its own
 //    32-bit sum (SDWA adds, v_add_u32 with src0_sel WORD_0 / WORD_1).
 //  * The staging lanes store each DM's LDS byte address (row window, copy
-//    and position) in the row's metadata; the summing lanes read it as a
-//    broadcast into a VGPR and add their lane offset, one add per row and
-//    DM.
+//    and position) in the row's metadata; the summing wavefront reads the
+//    words of a step's rows with one lane-distinct read, takes each into a
+//    scalar register (v_readlane) and adds its lane offset, one add per row
+//    and DM.
 #include "staged_common.hpp"
 
 namespace ddamd {

@@ -16,18 +16,16 @@
 //    once at the start.
 //  * Workgroups are ordered XCD-aware (staged_common.hpp).
 //
-// Byte rows are staged in four copies, copy c shifted down by c bytes (built
-// in registers with v_alignbyte), so that every four-sample group of a DM is
-// one aligned ds_read_b32.  Packed rows (inputBits 1, 2 or 4,
+// Rows are staged in four copies, copy c shifted down by c bytes (built in
+// registers with v_alignbyte), so that every four-sample group of a DM is one
+// aligned ds_read_b32.  Packed rows (inputBits 1, 2 or 4,
 // include/Dedispersion.hpp:92-103) are loaded as packed bytes and unpacked
-// to one byte per sample into a single copy; their summation reads two
-// dwords per group and aligns them with v_perm.
+// to one byte per sample in registers before the copies are built.
 //
 // The summation adds v_perm_b32-extracted 16-bit pairs, as kernels_u8.hip
-// does; partial sums are exact integers.  For byte rows each half-wavefront
-// sums one DM at a time (lane l: DM 2s + l / 32 of the wavefront's DMs, over
-// the groups (l % 32) + 32q), so every lane reads its own DM's LDS address;
-// packed rows keep lane l on the groups t0 + 4l + 256k of every DM.
+// does; partial sums are exact integers.  Each half-wavefront sums one DM at
+// a time (lane l: DM 2s + l / 32 of the wavefront's DMs, over the groups
+// (l % 32) + 32q), so every lane reads its own DM's LDS address.
 #include "staged_common.hpp"
 
 namespace ddamd {
@@ -35,60 +33,72 @@ namespace {
 
 using namespace staged;
 
-__device__ __forceinline__ uint32_t sel_even_of(uint32_t r) { return r | 0x0c00u | ((r + 2) << 16) | 0x0c000000u; }
-__device__ __forceinline__ uint32_t sel_odd_of(uint32_t r) { return (r + 1) | 0x0c00u | ((r + 3) << 16) | 0x0c000000u; }
-
-// Unpacks 16 bytes of packed samples (log2 of samples per byte = lf, 1..3)
-// into one byte per sample at dst (16 << lf bytes).  Samples are the fields
-// of each byte from the least significant bit up, so dword e of the chunk
-// holds samples (32 >> (3 - lf)) * e onwards, in field order.
-__device__ __forceinline__ void unpack_store(uint32_t *dst, const u32x4 &v, uint32_t lf) {
-  if (lf == 1) {
-    // 4-bit fields: even fields to bytes of `lo`, odd fields to bytes of `hi`,
-    // then interleaved
+// The 16 samples of window group m (one byte each, 4 dwords u) and the 4
+// after them (`next`) from the packed dwords d[0..2], which start at the
+// dword holding packed byte 16m >> LF; LF = log2 of samples per byte (1..3).
+// Samples are the fields of each byte from the least significant bit up.
+template <int LF>
+__device__ __forceinline__ void unpack_group(const u32x4 &d, uint32_t m, uint32_t (&u)[4], uint32_t &next) {
+  if constexpr (LF == 1) {
+    // 4-bit fields, 64 bits from d[0]: even fields to the bytes of `lo`, odd
+    // fields to those of `hi`, interleaved
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      u32x4 o;
-#pragma unroll
-      for (int e = 0; e < 2; ++e) {
-        const uint32_t d = v[2 * h + e];
-        const uint32_t lo = d & 0x0F0F0F0Fu, hi = (d >> 4) & 0x0F0F0F0Fu;
-        o[2 * e] = __builtin_amdgcn_perm(hi, lo, 0x05010400u);
-        o[2 * e + 1] = __builtin_amdgcn_perm(hi, lo, 0x07030602u);
+    for (int e = 0; e < 3; ++e) {
+      const uint32_t lo = d[e] & 0x0F0F0F0Fu, hi = (d[e] >> 4) & 0x0F0F0F0Fu;
+      const uint32_t a = __builtin_amdgcn_perm(hi, lo, 0x05010400u);
+      if (e < 2) {
+        u[2 * e] = a;
+        u[2 * e + 1] = __builtin_amdgcn_perm(hi, lo, 0x07030602u);
+      } else {
+        next = a;
       }
-      *reinterpret_cast<u32x4 *>(dst + 4 * h) = o;
     }
-  } else if (lf == 2) {
-    // 2-bit fields: m[j] byte k = sample 4k + j; two rounds of byte interleave
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const uint32_t d = v[e];
-      const uint32_t m0 = d & 0x03030303u, m1 = (d >> 2) & 0x03030303u;
-      const uint32_t m2 = (d >> 4) & 0x03030303u, m3 = (d >> 6) & 0x03030303u;
-      const uint32_t a01 = __builtin_amdgcn_perm(m1, m0, 0x05010400u);  // s0 s1 s4 s5
-      const uint32_t a23 = __builtin_amdgcn_perm(m3, m2, 0x05010400u);  // s2 s3 s6 s7
-      const uint32_t b01 = __builtin_amdgcn_perm(m1, m0, 0x07030602u);  // s8 s9 s12 s13
-      const uint32_t b23 = __builtin_amdgcn_perm(m3, m2, 0x07030602u);  // s10 s11 s14 s15
-      u32x4 o;
-      o[0] = __builtin_amdgcn_perm(a23, a01, 0x05040100u);
-      o[1] = __builtin_amdgcn_perm(a23, a01, 0x07060302u);
-      o[2] = __builtin_amdgcn_perm(b23, b01, 0x05040100u);
-      o[3] = __builtin_amdgcn_perm(b23, b01, 0x07060302u);
-      *reinterpret_cast<u32x4 *>(dst + 4 * e) = o;
-    }
+  } else if constexpr (LF == 2) {
+    // 2-bit fields, 32 bits from d[0]: m_j byte k = sample 4k + j; two
+    // rounds of byte interleave
+    const uint32_t x = d[0];
+    const uint32_t m0 = x & 0x03030303u, m1 = (x >> 2) & 0x03030303u;
+    const uint32_t m2 = (x >> 4) & 0x03030303u, m3 = (x >> 6) & 0x03030303u;
+    const uint32_t a01 = __builtin_amdgcn_perm(m1, m0, 0x05010400u);  // s0 s1 s4 s5
+    const uint32_t a23 = __builtin_amdgcn_perm(m3, m2, 0x05010400u);  // s2 s3 s6 s7
+    const uint32_t b01 = __builtin_amdgcn_perm(m1, m0, 0x07030602u);  // s8 s9 s12 s13
+    const uint32_t b23 = __builtin_amdgcn_perm(m3, m2, 0x07030602u);  // s10 s11 s14 s15
+    u[0] = __builtin_amdgcn_perm(a23, a01, 0x05040100u);
+    u[1] = __builtin_amdgcn_perm(a23, a01, 0x07060302u);
+    u[2] = __builtin_amdgcn_perm(b23, b01, 0x05040100u);
+    u[3] = __builtin_amdgcn_perm(b23, b01, 0x07060302u);
+    const uint32_t y = d[1];
+    const uint32_t c01 = __builtin_amdgcn_perm((y >> 2) & 0x03030303u, y & 0x03030303u, 0x05010400u);
+    const uint32_t c23 = __builtin_amdgcn_perm((y >> 6) & 0x03030303u, (y >> 4) & 0x03030303u, 0x05010400u);
+    next = __builtin_amdgcn_perm(c23, c01, 0x05040100u);
   } else {
-    // 1-bit fields: a nibble n spread to bytes by n * (1 + 2^7 + 2^14 + 2^21),
-    // whose four shifted copies do not overlap, then bit 0 of each byte kept
+    // 1-bit fields, 16 bits from bit 16 (m & 1) of d[0]: a nibble n spread to
+    // bytes by n * (1 + 2^7 + 2^14 + 2^21), whose four shifted copies do not
+    // overlap, then bit 0 of each byte kept
+    const uint32_t x = __builtin_amdgcn_alignbit(d[1], d[0], 16 * (m & 1));
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
+    for (int q = 0; q < 4; ++q) u[q] = (((x >> (4 * q)) & 0xFu) * 0x204081u) & 0x01010101u;
+    next = (((x >> 16) & 0xFu) * 0x204081u) & 0x01010101u;
+  }
+}
+
+// Writes the byte run u (n dwords, followed by the dword `next`) to the
+// four copies of a row window at dword position `at`: copy c holds the run
+// shifted down by c bytes.
+template <int N>
+__device__ __forceinline__ void store_copies(uint32_t *w, uint32_t WD, uint32_t at, const uint32_t (&u)[N],
+                                             uint32_t next) {
+  static_assert(N % 4 == 0, "whole 16-byte groups");
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        u32x4 o;
+  for (int i = 0; i < N; i += 4) {
+    *reinterpret_cast<u32x4 *>(w + at + i) = u32x4{u[i], u[i + 1], u[i + 2], u[i + 3]};
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-          o[q] = (((v[e] >> (16 * h + 4 * q)) & 0xFu) * 0x204081u) & 0x01010101u;
-        *reinterpret_cast<u32x4 *>(dst + 8 * e + 4 * h) = o;
-      }
+    for (int c = 1; c < 4; ++c) {
+      u32x4 r;
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        r[e] = __builtin_amdgcn_alignbyte(i + e + 1 < N ? u[i + e + 1] : next, u[i + e], c);
+      *reinterpret_cast<u32x4 *>(w + c * WD + at + i) = r;
     }
   }
 }
@@ -119,19 +129,17 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
   const uint32_t WD = p.win_bytes / 4;  // dwords per window
   const uint32_t NCH = WD / 4;          // 16-byte chunks per window
   const uint32_t lf = PACKED ? p.in_log2f : 0u;  // log2 of samples per input byte
-  const uint32_t NCH_IN = NCH >> lf;    // 16-byte chunks of packed input per window
   const uint32_t dmax = p.n_dms - 1;
 
-  // LDS: rows (packed row | beam << 16, float delay) | windows [2][G][COPIES][WD] | DM words [2][G][TD]
-  // Byte rows are kept in four copies, copy c shifted down by c bytes, so a
-  // DM's four-sample group at any offset is one aligned dword of copy
-  // (offset & 3); packed rows are unpacked into one copy.
-  constexpr uint32_t COPIES = PACKED ? 1 : 4;
+  // LDS: rows (packed row | beam << 16, float delay) | windows [2][G][4][WD] | DM words [2][G][TD]
+  // Rows are kept in four copies, copy c shifted down by c bytes, so a DM's
+  // four-sample group at any offset is one aligned dword of copy
+  // (offset & 3); packed rows are unpacked to one byte per sample first.
   const uint32_t win_off = (2 * p.live_pitch + 3) & ~3u;
   uint32_t *lrow = lds;
   float *ltab = reinterpret_cast<float *>(lds + p.live_pitch);
   uint32_t *windows = lds + win_off;
-  uint32_t *meta = windows + 2 * G * COPIES * WD;
+  uint32_t *meta = windows + 2 * G * 4 * WD;
 
   for (uint32_t k = threadIdx.x; k < nlive; k += blockDim.x) {
     const uint32_t row = live[k];
@@ -143,7 +151,7 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
 
   // ---- staging: wavefront w stages rows w and w + nwaves of a group ----
   u32x4 sv[SPW][CPL];
-  uint32_t sx[SPW][CPL];  // the dword after each chunk (byte rows)
+  uint32_t sx[SPW][CPL];  // the (packed) dword after each chunk
   uint32_t so[SPW];
   auto stage_load = [&](uint32_t k0, uint32_t nn) {
 #pragma unroll
@@ -168,14 +176,30 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
           const uint32_t m = lane + 64 * c;
-          if (m < NCH_IN) {
+          if (m >= NCH) continue;
+          if (!PACKED || lf == 0) {
+            // 16 bytes of the window and the dword after them
             if (16 * m + 16 <= recs) {
               sv[s][c] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, 16 * m, 0, 0);
             } else {
 #pragma unroll
               for (int e = 0; e < 4; ++e) sv[s][c][e] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, 16 * m + 4 * e, 0, 0);
             }
-            if constexpr (!PACKED) sx[s][c] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, 16 * m + 16, 0, 0);
+            sx[s][c] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, 16 * m + 16, 0, 0);
+          } else {
+            // the packed bytes of the window's 16 samples from 16m and of the
+            // 4 after them: three dwords from the dword that holds packed
+            // byte 16m >> lf
+            const uint32_t pd = (16 * m >> lf) & ~3u;
+            if (pd + 12 <= recs) {
+              const u32x3 t = __builtin_amdgcn_raw_buffer_load_b96(rsrc, pd, 0, 0);
+              sv[s][c][0] = t[0];
+              sv[s][c][1] = t[1];
+              sv[s][c][2] = t[2];
+            } else {
+#pragma unroll
+              for (int e = 0; e < 3; ++e) sv[s][c][e] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, pd + 4 * e, 0, 0);
+            }
           }
         }
       }
@@ -186,47 +210,37 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
     for (int s = 0; s < SPW; ++s) {
       const uint32_t g = wave + s * nwaves;
       if (g < nn) {
-        uint32_t *w = windows + (buf * G + g) * COPIES * WD;
-        if constexpr (!PACKED) {
+        uint32_t *w = windows + (buf * G + g) * 4 * WD;
+        auto unpacked = [&](auto lfc) {
+          constexpr int LF = lfc.value;
 #pragma unroll
           for (int c = 0; c < CPL; ++c) {
             const uint32_t m = lane + 64 * c;
             if (m < NCH) {
-              const u32x4 v = sv[s][c];
-              const uint32_t nx = sx[s][c];
-              u32x4 r1, r2, r3;
-#pragma unroll
-              for (int e = 0; e < 4; ++e) {
-                const uint32_t hi = e < 3 ? v[e + 1] : nx;
-                r1[e] = __builtin_amdgcn_alignbyte(hi, v[e], 1);
-                r2[e] = __builtin_amdgcn_alignbyte(hi, v[e], 2);
-                r3[e] = __builtin_amdgcn_alignbyte(hi, v[e], 3);
-              }
-              *reinterpret_cast<u32x4 *>(w + 4 * m) = v;
-              *reinterpret_cast<u32x4 *>(w + WD + 4 * m) = r1;
-              *reinterpret_cast<u32x4 *>(w + 2 * WD + 4 * m) = r2;
-              *reinterpret_cast<u32x4 *>(w + 3 * WD + 4 * m) = r3;
+              uint32_t u[4];
+              uint32_t next;
+              unpack_group<LF>(sv[s][c], m, u, next);
+              store_copies(w, WD, 4 * m, u, next);
             }
           }
-          // the DM's LDS byte address: copy (offset & 3), dword offset >> 2
-          const uint32_t o = so[s];
-          if (lane < TD) meta[(buf * G + g) * TD + lane] = 4 * (win_off + ((buf * G + g) * 4 + (o & 3)) * WD + (o >> 2));
-        } else {
-          if (lf == 0) {
+        };
+        if (!PACKED || lf == 0) {
 #pragma unroll
-            for (int c = 0; c < CPL; ++c) {
-              const uint32_t m = lane + 64 * c;
-              if (m < NCH) *reinterpret_cast<u32x4 *>(w + 4 * m) = sv[s][c];
-            }
-          } else {
-#pragma unroll
-            for (int c = 0; c < CPL; ++c) {
-              const uint32_t m = lane + 64 * c;
-              if (m < NCH_IN) unpack_store(w + ((4 * m) << lf), sv[s][c], lf);
+          for (int c = 0; c < CPL; ++c) {
+            const uint32_t m = lane + 64 * c;
+            if (m < NCH) {
+              const uint32_t u[4] = {sv[s][c][0], sv[s][c][1], sv[s][c][2], sv[s][c][3]};
+              store_copies(w, WD, 4 * m, u, sx[s][c]);
             }
           }
-          if (lane < TD) meta[(buf * G + g) * TD + lane] = so[s];
+        } else if constexpr (PACKED) {
+          if (lf == 1) unpacked(std::integral_constant<int, 1>{});
+          else if (lf == 2) unpacked(std::integral_constant<int, 2>{});
+          else unpacked(std::integral_constant<int, 3>{});
         }
+        // the DM's LDS byte address: copy (offset & 3), dword offset >> 2
+        const uint32_t o = so[s];
+        if (lane < TD) meta[(buf * G + g) * TD + lane] = 4 * (win_off + ((buf * G + g) * 4 + (o & 3)) * WD + (o >> 2));
       }
     }
   };
@@ -262,7 +276,7 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
   // Rows are taken two at a time so that the two extracted pairs of a DM
   // and sample group go into its accumulator with one three-input add.
   //
-  // Byte rows: each half-wavefront takes one DM at a time.  With LPD = 32
+  // Each half-wavefront takes one DM at a time.  With LPD = 32
   // lanes per DM (64 when the wavefront has one DM), lane l sums, for DM
   // slot s < SL, DM dw + s CPS + l / LPD over the four-sample groups
   // (l % LPD) + LPD q, q < NGS: the 32 lanes of a half read 32 consecutive
@@ -328,59 +342,7 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
       }
     }
   };
-  // Packed rows (one copy) keep lane l on the groups t0 + 4l + 256k of every
-  // DM of the wavefront: the DM words of two rows reach the wavefront with
-  // one LDS read (lane l < 2 DPW reads the word of row l / DPW, DM l % DPW)
-  // and are taken from their lanes into scalar registers (v_readlane).
-  const uint32_t lane_row = (lane / DPW) & 1, lane_word = lane % DPW;
-  auto words_of = [&](uint32_t buf, uint32_t g, uint32_t rows) {
-    return meta[(buf * G + g + (rows > 1 ? lane_row : 0u)) * TD + dw + lane_word];
-  };
-  auto word = [&](uint32_t words, int i) { return (uint32_t)__builtin_amdgcn_readlane((int)words, i); };
-  // A packed row's group spans two dwords from its word's dword offset,
-  // aligned by the v_perm selectors of its byte offset.
-  auto compute = [&](uint32_t buf, uint32_t n) {
-    if constexpr (!PACKED) {
-      compute_bytes(buf, n);
-      return;
-    }
-    uint32_t g = 0;
-    for (; g + 1 < n; g += 2) {
-      const uint32_t words = words_of(buf, g, 2);
-      const uint32_t *win0 = windows + (buf * G + g) * WD + lane;
-      const uint32_t *win1 = win0 + WD;
-#pragma unroll
-      for (int j = 0; j < DPW; ++j) {
-        const uint32_t o0 = word(words, j), o1 = word(words, DPW + j);
-        const uint32_t *src0 = win0 + (o0 >> 2), *src1 = win1 + (o1 >> 2);
-        const uint32_t se0 = sel_even_of(o0 & 3), sd0 = sel_odd_of(o0 & 3);
-        const uint32_t se1 = sel_even_of(o1 & 3), sd1 = sel_odd_of(o1 & 3);
-#pragma unroll
-        for (int k = 0; k < NG; ++k) {
-          const uint32_t lo0 = src0[64 * k], hi0 = src0[64 * k + 1];
-          const uint32_t lo1 = src1[64 * k], hi1 = src1[64 * k + 1];
-          even[j][k] += __builtin_amdgcn_perm(hi0, lo0, se0) + __builtin_amdgcn_perm(hi1, lo1, se1);
-          odd[j][k] += __builtin_amdgcn_perm(hi0, lo0, sd0) + __builtin_amdgcn_perm(hi1, lo1, sd1);
-        }
-      }
-    }
-    if (g < n) {
-      const uint32_t words = words_of(buf, g, 1);
-      const uint32_t *win = windows + (buf * G + g) * WD + lane;
-#pragma unroll
-      for (int j = 0; j < DPW; ++j) {
-        const uint32_t o = word(words, j);
-        const uint32_t *src = win + (o >> 2);
-        const uint32_t se = sel_even_of(o & 3), sd = sel_odd_of(o & 3);
-#pragma unroll
-        for (int k = 0; k < NG; ++k) {
-          const uint32_t lo = src[64 * k], hi = src[64 * k + 1];
-          even[j][k] += __builtin_amdgcn_perm(hi, lo, se);
-          odd[j][k] += __builtin_amdgcn_perm(hi, lo, sd);
-        }
-      }
-    }
-  };
+  auto compute = [&](uint32_t buf, uint32_t n) { compute_bytes(buf, n); };
 
   // ---- pipeline over groups of rows ----
   uint32_t cur = 0;
@@ -411,10 +373,8 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
   flush();
 
   // the DM and first sample of accumulator [j][k]
-  auto dm_of = [&](int j) -> uint32_t { return PACKED ? d0 + dw + j : d0 + dw + (j / CPS) * CPS + lane_dm; };
-  auto t_of = [&](int j, int k) -> uint32_t {
-    return PACKED ? t0 + 4 * lane + 256 * k : t0 + 4 * (lane % LPD) + 4 * LPD * ((j % CPS) * NG + k);
-  };
+  auto dm_of = [&](int j) -> uint32_t { return d0 + dw + (j / CPS) * CPS + lane_dm; };
+  auto t_of = [&](int j, int k) -> uint32_t { return t0 + 4 * (lane % LPD) + 4 * LPD * ((j % CPS) * NG + k); };
   if (p.out_u16) {
     // compact output: the four sums of a group as uint16 (each below 2^16)
 #pragma unroll
@@ -481,17 +441,18 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
 }
 
 // Shapes whose sums fit in 64 VGPRs (RUN x DPW <= 128, no 32-bit flush
-// sums) are held to four waves per SIMD: the loads issued one DM ahead and
+// sums) are held to four waves per SIMD: the loads issued a chunk ahead and
 // the queue loop of the persistent form can otherwise push the allocation
-// just past 128 and cost a wave.  The per-tile kernels of packed rows are
-// left to the compiler (held to 128 VGPRs, one of them spills).
-template <int RUN, int DPW, bool FLUSH, bool PACKED, bool PER_TILE>
+// just past 128 and cost a wave.  Kernels that also read packed rows need
+// registers for the unpacking and are held to four waves only up to
+// RUN x DPW = 64.
+template <int RUN, int DPW, bool FLUSH, bool PACKED>
 constexpr int u8_min_waves() {
-  return (!FLUSH && !(PACKED && PER_TILE) && RUN * DPW <= 128) ? 4 : 1;
+  return (!FLUSH && RUN * DPW <= (PACKED ? 64 : 128)) ? 4 : 1;
 }
 
 template <int RUN, int DPW, bool FLUSH, bool PACKED>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(u8_min_waves<RUN, DPW, FLUSH, PACKED, true>())))
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(u8_min_waves<RUN, DPW, FLUSH, PACKED>())))
 rowsum_u8_staged(const RowSumU8Params p) {
   extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
   u8_tile<RUN, DPW, FLUSH, PACKED>(p, xcd_logical(blockIdx.x, p.n_tiles), lds);
@@ -500,7 +461,7 @@ rowsum_u8_staged(const RowSumU8Params p) {
 // Persistent form: workgroups take (beam, tile) descriptors from XCD-local
 // work queues until every queue is empty (see staged_common.hpp).
 template <int RUN, int DPW, bool FLUSH, bool PACKED>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(u8_min_waves<RUN, DPW, FLUSH, PACKED, false>())))
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(u8_min_waves<RUN, DPW, FLUSH, PACKED>())))
 rowsum_u8_batch(const RowSumU8Params *__restrict__ beams, const TileQueues q) {
   extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
   __shared__ uint32_t claimed;
@@ -515,7 +476,7 @@ rowsum_u8_batch(const RowSumU8Params *__restrict__ beams, const TileQueues q) {
 
 // One workgroup per listed (job, tile) descriptor, in XCD-aware order.
 template <int RUN, int DPW, bool FLUSH, bool PACKED>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(u8_min_waves<RUN, DPW, FLUSH, PACKED, false>())))
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(u8_min_waves<RUN, DPW, FLUSH, PACKED>())))
 rowsum_u8_list(const RowSumU8Params *__restrict__ jobs, const TileDesc *__restrict__ tiles, uint32_t n) {
   extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
   const TileDesc d = tiles[xcd_logical(blockIdx.x, n)];

@@ -337,13 +337,10 @@ dd_status geometry_of(const dd_plan *p, dd_mode mode, const dd_conf &c, Geometry
       g.lds_bytes = c.unroll * g.win;
       if (stageable && g.win <= 16 * 64 * kU8StagedChunks) {
         const uint32_t live_pitch = mode == DD_SINGLE_STEP ? p->obs.nr_channels : cps(p);
-        // rows | [2 buffers][unroll rows][copies][win / 4 dwords] | DM words; byte
-        // rows are kept in four shifted copies, packed rows in one; rows
-        // staged per barrier are capped at what fits in LDS
-        const uint32_t copies = per > 1 ? 1 : 4;
-        auto lds_of = [&](uint32_t u) {
-          return 4 * (roundup(2 * live_pitch, 4) + 2 * u * copies * (g.win / 4) + 2 * u * tile);
-        };
+        // rows | [2 buffers][unroll rows][4 shifted copies][win / 4 dwords] |
+        // DM words (packed rows are unpacked before the copies are built);
+        // rows staged per barrier are capped at what fits in LDS
+        auto lds_of = [&](uint32_t u) { return 4 * (roundup(2 * live_pitch, 4) + 2 * u * 4 * (g.win / 4) + 2 * u * tile); };
         uint32_t unroll = c.unroll;
         while (unroll > 1 && lds_of(unroll) > p->max_lds) --unroll;
         g.staged = 1;

@@ -5,8 +5,9 @@ Inputs are random values over the whole field (top bit included), packed on
 the host by tests/packing.py; the device unpacks them while staging rows.
 Cases: several beams with a mixed beam mapping and zapped channels, padded
 and unpadded rows (odd byte offsets), every kernel selection (local 0, 1, 3
-all run the pipelined kernel for packed rows), the two-step path, an
-ARTS-sized two-step beam, and batches mixing packed and 8-bit jobs."""
+all run the pipelined kernel for packed rows), more than 257 live channels
+(the kernels with 32-bit flush sums), the two-step path, an ARTS-sized
+two-step beam, and batches mixing packed and 8-bit jobs."""
 import numpy as np
 import pytest
 
@@ -66,6 +67,29 @@ def test_single_step_packed(torch_cuda, bits, padding, conf):
     torch.cuda.synchronize()
     want = orc.dedispersion(host, flags, mapping)
     compare_rows(d_out.cpu().numpy().reshape(want.shape), want, z.nr_samples, f"{bits}-bit single step")
+    plan.close()
+
+
+@pytest.mark.parametrize("conf", [Conf(64, 2, 16, 8, 4, 1), Conf(64, 4, 8, 2, 4, 1)], ids=["16x8", "8x2"])
+@pytest.mark.parametrize("bits", BITS, ids=lambda b: f"{b}bit")
+def test_single_step_packed_many_channels(torch_cuda, bits, conf):
+    """More than 257 live channels: the kernels that flush their 16-bit pair
+    sums into 32-bit sums every 256 rows, over packed rows."""
+    torch = torch_cuda
+    w = workloads.tiny(channels=320, samples=1100, dms=16, step=0.4, beams=1, sbeams=1, min_freq=1300.0,
+                       bandwidth=0.5, zapped=(3, 200), seed=43)
+    plan = Plan(w.obs, 128, input_bits=bits)
+    orc = oracle.Oracle(w.obs, 128, threads=host_threads(), input_bits=bits)
+    z = plan.sizes
+    flags = zapped_flags(plan, w.zapped)
+    plan.set_zapped_channels(flags)
+    plan.set_conf(DD_SINGLE_STEP, conf)
+    host, d_in = packed_input(torch, orc, bits, z.nr_dispersed_samples, z.in_pitch, seed=10 + bits)
+    d_out = torch.full((z.output_elements,), -1.0, dtype=torch.float32, device="cuda")
+    plan.dedisperse(d_in.data_ptr(), d_out.data_ptr(), stream=stream_of(torch))
+    torch.cuda.synchronize()
+    want = orc.dedispersion(host, flags)
+    compare_rows(d_out.cpu().numpy().reshape(want.shape), want, z.nr_samples, f"{bits}-bit single step, 320 channels")
     plan.close()
 
 

@@ -40,8 +40,9 @@ using namespace staged;
 template <int LF>
 __device__ __forceinline__ void unpack_group(const u32x4 &d, uint32_t m, uint32_t (&u)[4], uint32_t &next) {
   if constexpr (LF == 1) {
-    // 4-bit fields, 64 bits from d[0]: even fields to the bytes of `lo`, odd
-    // fields to those of `hi`, interleaved
+    // 4-bit fields, 64 bits from d[0] and d[1] (the next 4 samples in the low
+    // half of d[2]): even fields to the bytes of `lo`, odd fields to those
+    // of `hi`, interleaved
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
       const uint32_t lo = d[e] & 0x0F0F0F0Fu, hi = (d[e] >> 4) & 0x0F0F0F0Fu;
@@ -54,8 +55,8 @@ __device__ __forceinline__ void unpack_group(const u32x4 &d, uint32_t m, uint32_
       }
     }
   } else if constexpr (LF == 2) {
-    // 2-bit fields, 32 bits from d[0]: m_j byte k = sample 4k + j; two
-    // rounds of byte interleave
+    // 2-bit fields, 32 bits from d[0] (the next 4 samples in the low byte of
+    // d[1]): m_j byte k = sample 4k + j; two rounds of byte interleave
     const uint32_t x = d[0];
     const uint32_t m0 = x & 0x03030303u, m1 = (x >> 2) & 0x03030303u;
     const uint32_t m2 = (x >> 4) & 0x03030303u, m3 = (x >> 6) & 0x03030303u;
@@ -151,7 +152,7 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
 
   // ---- staging: wavefront w stages rows w and w + nwaves of a group ----
   u32x4 sv[SPW][CPL];
-  uint32_t sx[SPW][CPL];  // the (packed) dword after each chunk
+  uint32_t sx[SPW][CPL];  // the dword after each 16-byte chunk (byte rows)
   uint32_t so[SPW];
   auto stage_load = [&](uint32_t k0, uint32_t nn) {
 #pragma unroll

@@ -354,6 +354,14 @@ dd_status geometry_of(const dd_plan *p, dd_mode mode, const dd_conf &c, Geometry
       }
     }
   }
+  if (g.local && !g.staged && g.lds_bytes > p->max_lds) {
+    // the rows' windows (tile plus the spread of the tile's shifts) do not
+    // fit in LDS: the direct kernel, same shape, reads the rows through the
+    // caches (the same sums)
+    g.local = 0;
+    g.win = 0;
+    g.lds_bytes = 0;
+  }
   if (g.lds_bytes > p->max_lds)
     return fail(DD_ERR_CONFIGURATION, "LDS demand " + std::to_string(g.lds_bytes) + " bytes exceeds the device limit");
   return DD_OK;

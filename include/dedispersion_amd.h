@@ -83,7 +83,9 @@ typedef struct dd_observation {
  *               items_d1 4 or 8, items_d0 8 or 16, unroll (blocks per
  *               phase) <= 8; plans whose shift table has more than 16
  *               patterns per block and tile, or whose phase of `unroll`
- *               blocks does not fit in LDS, run the staged pipeline
+ *               blocks does not fit in LDS, run the staged pipeline.
+ *               Local 1 and 2 whose row windows (the tile plus the spread
+ *               of its DMs' shifts) do not fit in LDS run as local 0.
  *   split_batches must be 0 (unimplemented in the reference too). */
 typedef struct dd_conf {
   uint32_t threads_d0;

@@ -1,7 +1,8 @@
 """Degenerate inputs on the device against the oracle, bit for bit: every
 channel zapped (every sum is zero, as the reference's loops over no channel
 leave it), a subband whose channels are all zapped, minimal shapes (one DM,
-one output sample, two channels), and a batch with no jobs."""
+one output sample, two channels), a batch with no jobs, and shift spreads
+too wide for any LDS window."""
 import numpy as np
 import pytest
 
@@ -99,3 +100,32 @@ def test_empty_batch(torch_cuda):
         for mode in (DD_SINGLE_STEP, DD_STEP_ONE, DD_STEP_TWO):
             b.run_mode(mode)
     torch_cuda.cuda.synchronize()
+
+
+def test_single_step_wide_shift_spread(torch_cuda):
+    """300 MHz at the bottom of a 128 MHz band and DM steps of 20: the shifts
+    of neighbouring DMs are thousands of samples apart, far wider than any
+    LDS window.  Every configuration runs, the LDS ones as the direct kernel
+    of the same shape (rows read through the caches), and is exact."""
+    w = workloads.tiny(channels=64, samples=2000, dms=12, step=20.0, min_freq=300.0, bandwidth=2.0,
+                       zapped=(5,), seed=12)
+    for conf in (None, Conf(64, 2, 8, 2, 4, 0), Conf(64, 4, 16, 8, 4, 1), Conf(64, 2, 8, 4, 2, 2)):
+        plan, inp, out, orc, flags = single_step(torch_cuda, w, conf=conf)
+        assert plan.sizes.nr_dispersed_samples > 30000
+        want = orc.dedispersion(inp, flags)
+        compare_rows(out.reshape(want.shape), want, plan.sizes.nr_samples, f"wide spread {conf}")
+        plan.close()
+
+
+def test_two_step_wide_shift_spread(torch_cuda):
+    """The same band in the two-step path: coarse DM steps of 40 spread step
+    one's shifts beyond its LDS window, fine steps of 10 spread step two's."""
+    w = workloads.tiny(channels=64, samples=1000, dms=4, step=10.0, subbands=4, coarse=4, coarse_step=40.0,
+                       min_freq=300.0, bandwidth=2.0, zapped=(5,), seed=13)
+    plan, flags, inp, mid, out, orc = two_step(torch_cuda, w)
+    z = plan.sizes
+    want_mid = orc.step_one(inp, flags)
+    compare_rows(mid.reshape(want_mid.shape), want_mid, z.nr_samples_subbanding, "step one")
+    want = orc.step_two(mid)
+    compare_rows(out.reshape(want.shape), want, w.obs.nr_samples_per_batch, "step two")
+    plan.close()

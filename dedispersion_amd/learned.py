@@ -8,7 +8,7 @@ features, three ReLU layers of 64 units, one output, FP32 -- are trained on
 those records by the library's own HIP kernels (``dd_mlp_*``), predict the
 run time of every configuration of a new layout, and the tuner times only
 the best-predicted fraction.  The exhaustive sweep stays the fallback (no
-model, or a budget of 1) and the judge (tests compare with it).
+model, or a budget of 1) and the yardstick (tests compare with it).
 
 Features of a (layout, configuration) pair:
   threads_d0, log2 threads_d1, log2 items_d0, log2 items_d1, log2 unroll,

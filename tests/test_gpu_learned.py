@@ -3,7 +3,7 @@ on the exhaustive-sweep records of the 128-channel, ARTS single-step and
 ARTS two-step layouts (2880 timed configurations), then ranks the
 configurations of layouts it has not seen -- the shapes of the 40-beam
 workload -- and only the best-predicted 10% are timed.  The exhaustive sweep
-is the judge: the pick must be within 5% of its optimum, for at most 10% of
+is the yardstick: the pick must be within 5% of its optimum, for at most 10% of
 its timing cost, and must beat the same budget spent without the model."""
 import numpy as np
 import pytest

@@ -457,7 +457,7 @@ def run_single_plan(args, torch):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU per process; multi-GPU runs are not part of this build",
+        print(json.dumps({"error": "this engine runs on one GPU per process; runs over several GPUs are not supported",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     import torch

@@ -2,7 +2,7 @@
 // reads.  The reference takes it from the external AstroData library
 // (AstroData::Observation, used at include/Dedispersion.hpp:72-171 and
 // src/Shifts.cpp:21-40 of TRASAL/Dedispersion); that library is not part of
-// the reference tree, so this build owns a compatible type with the getters
+// the reference tree, so this library owns a compatible type with the getters
 // and setters the path uses, and states their semantics here:
 //
 //   getMaxFreq()          = minFreq + (C - 1) * channelBandwidth

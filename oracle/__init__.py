@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (dd_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py as the checker; the product never uses it.
+cpu_baseline leg of bench.py to check results; the library never uses it.
 """
 from __future__ import annotations
 

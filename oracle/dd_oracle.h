@@ -2,8 +2,8 @@
  * dd_oracle.h -- CPU oracle for the dedispersion hot path.
  *
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the
- * reference's CPU algorithm (TRASAL/Dedispersion) used as the checker for
- * the HIP kernels.  Only tests/, __graft_entry__.smoke() and the
+ * reference's CPU algorithm (TRASAL/Dedispersion) against which the HIP
+ * kernels are checked.  Only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py load it.  The product library never links or
  * calls it.
  *
@@ -70,7 +70,7 @@ typedef struct orc_obs {
 /* isa::utils::pad: round v up to a multiple of k; k == 0 means no padding. */
 uint32_t orc_pad(uint32_t v, uint32_t k);
 
-/* Derived frequencies (AstroData semantics as defined by this build). */
+/* Derived frequencies (AstroData semantics as defined here). */
 float orc_max_freq(const orc_obs *o);
 float orc_subband_bandwidth(const orc_obs *o);
 float orc_subband_min_freq(const orc_obs *o);

@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/dd_oracle.c) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/dd_oracle.c) before the device results are checked against it.
 
 The reference ships no golden vectors (its only known answer is a constant
 input, src/DedispersionTest.cpp:213, 260, 304), so each oracle function is

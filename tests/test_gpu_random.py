@@ -1,0 +1,146 @@
+"""Seeded random shapes on the device against the oracle, bit for bit: the
+single step, the two-step path and batches of uneven two-step jobs, each
+over shapes drawn from a splitmix64 stream (channels, samples, DMs, DM steps,
+band, zapped channels, beams and their mapping, padding) and a configuration
+drawn from the valid ones.  The fixed-shape tests sweep configurations; these
+sweep shapes."""
+import numpy as np
+import pytest
+
+import oracle
+from dedispersion_amd import DD_STEP_ONE, DD_STEP_TWO, Batch, Conf, DDError, workloads
+from dedispersion_amd.workloads import SplitMix64
+from tests.gpu_helpers import compare_rows, single_step, stream_of
+from tests.test_gpu_batch import Jobs
+from tests.test_gpu_parity import STEP_ONE_CONFS, STEP_TWO_CONFS, SWEEP_CONFS, two_step_case
+
+pytestmark = pytest.mark.gpu
+
+PADDINGS = (0, 32, 128)
+
+
+def uniform(rng: SplitMix64, lo: float, hi: float) -> float:
+    return lo + (hi - lo) * rng.below(1 << 20) / float(1 << 20)
+
+
+def zapped_of(rng: SplitMix64, channels: int):
+    n = rng.below(channels // 5 + 1)
+    return tuple(sorted(rng.sample(channels, n))) if n else ()
+
+
+def single_shape(seed: int):
+    rng = SplitMix64(9000 + seed)
+    channels = 2 + rng.below(399)
+    beams = 1 + rng.below(2)
+    return dict(channels=channels, samples=1 + rng.below(3000), dms=1 + rng.below(80),
+                step=uniform(rng, 0.0, 2.0), min_freq=uniform(rng, 1100.0, 1500.0),
+                bandwidth=uniform(rng, 0.2, 1.5), beams=beams, sbeams=beams + rng.below(2),
+                zapped=zapped_of(rng, channels), seed=seed), PADDINGS[rng.below(3)], rng.below(len(SWEEP_CONFS) + 1)
+
+
+def two_step_shape(seed: int):
+    rng = SplitMix64(7000 + seed)
+    subbands = (1, 2, 4, 8, 16)[rng.below(5)]
+    channels = subbands * (1 + rng.below(40))
+    beams = 1 + rng.below(2)
+    return dict(channels=channels, samples=1 + rng.below(2000), dms=1 + rng.below(40), step=uniform(rng, 0.0, 1.5),
+                subbands=subbands, coarse=1 + rng.below(12), coarse_step=uniform(rng, 0.0, 8.0),
+                min_freq=uniform(rng, 1100.0, 1500.0), bandwidth=uniform(rng, 0.2, 1.5), beams=beams,
+                sbeams=beams + rng.below(2), zapped=zapped_of(rng, channels), seed=seed), \
+        PADDINGS[rng.below(3)], rng.below(len(STEP_ONE_CONFS) + 1), rng.below(len(STEP_TWO_CONFS) + 1)
+
+
+def pick(confs, i):
+    """Index len(confs) stands for the plan default."""
+    return None if i == len(confs) else confs[i]
+
+
+def u16_window_fits(w, conf: Conf) -> bool:
+    """Whether a batch's uint16 step two can stage this job's rows: a row
+    window is the sample tile (64 items_d0) plus the largest spread of the
+    step-two shifts over a tile of threads_d1 x items_d1 DMs (a tile past the
+    last DM ends at the last DM), plus 8 elements of alignment slack, and a
+    lane stages at most items_d0 / 8 + 1 loads of 8 elements.  Shifts from
+    the oracle."""
+    o = w.obs
+    orc = oracle.Oracle(o, 0, threads=1)
+    table = orc.get_shifts_step_two()
+    tile = conf.threads_d1 * conf.items_d1
+    spread = max(orc.shift(o.first_dm, o.dm_step, min(d0 + tile, o.nr_dms) - 1, table[sb]) -
+                 orc.shift(o.first_dm, o.dm_step, d0, table[sb])
+                 for sb in range(o.nr_subbands) for d0 in range(0, o.nr_dms, tile))
+    win = (64 * conf.items_d0 + spread + 8 + 7) // 8 * 8
+    return win // 8 <= 64 * (conf.items_d0 // 8 + 1)
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_single_step_random_shape(torch_cuda, seed):
+    shape, padding, ci = single_shape(seed)
+    w = workloads.tiny(**shape)
+    conf = pick(SWEEP_CONFS, ci)
+    plan, inp, out, orc, flags = single_step(torch_cuda, w, padding=padding, conf=conf)
+    want = orc.dedispersion(inp, flags)
+    got = out.reshape(want.shape)
+    compare_rows(got, want, plan.sizes.nr_samples, f"seed {seed} {shape} padding {padding} conf {conf}")
+    assert np.all(got[..., w.obs.nr_samples_per_batch:] == -1.0)
+    plan.close()
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_two_step_random_shape(torch_cuda, seed):
+    shape, padding, c1, c2 = two_step_shape(seed)
+    w = workloads.tiny(**shape)
+    conf1, conf2 = pick(STEP_ONE_CONFS, c1), pick(STEP_TWO_CONFS, c2)
+    plan, flags, inp, mid, out, orc = two_step_case(torch_cuda, w, padding, conf1, conf2)
+    z = plan.sizes
+    what = f"seed {seed} {shape} padding {padding} confs {conf1} / {conf2}"
+    want_mid = orc.step_one(inp, flags)
+    compare_rows(mid.reshape(want_mid.shape), want_mid, z.nr_samples_subbanding, "step one, " + what)
+    want = orc.step_two(mid)
+    compare_rows(out.reshape(want.shape), want, w.obs.nr_samples_per_batch, "step two, " + what)
+    plan.close()
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_batch_random_jobs(torch_cuda, seed):
+    """3 to 6 random two-step jobs in one batch, uint16 intermediate, a random
+    pipeline depth and staged configurations, against the oracle and against
+    per-job launches.  Jobs too wide for the step-two staging window are first
+    shown to be refused, then left out."""
+    rng = SplitMix64(5000 + seed)
+    ws = [workloads.tiny(**two_step_shape(100 * seed + j)[0]) for j in range(3 + rng.below(4))]
+    chunks = (0, 2, 3)[rng.below(3)]
+
+    def staged_conf():
+        t1 = (1, 2, 4)[rng.below(3)]
+        # a batch stages at most two rows per wavefront per group: unroll <= 2 threads_d1
+        return Conf(64, t1, (8, 16)[rng.below(2)], (2, 4, 8)[rng.below(3)], 1 + rng.below(min(4, 2 * t1)), 1)
+
+    one, two = staged_conf(), staged_conf()
+    fits = [u16_window_fits(w, two) for w in ws]
+    if not all(fits):
+        # a job whose step-two rows do not fit the staging window is refused
+        # with a configuration error, not run
+        jobs = Jobs(torch_cuda, ws)
+        with Batch() as b:
+            b.set_conf(DD_STEP_ONE, one)
+            b.set_conf(DD_STEP_TWO, two)
+            for plan, d_in, d_mid, d_out in zip(jobs.plans, jobs.inputs, jobs.mids, jobs.outs):
+                b.add_two_step(plan, d_in.data_ptr(), d_mid.data_ptr(), d_out.data_ptr())
+            with pytest.raises(DDError, match="shift spread too large for the uint16 window"):
+                b.run(use_graph=False, stream=stream_of(torch_cuda))
+        ws = [w for w, f in zip(ws, fits) if f]
+    jobs = Jobs(torch_cuda, ws)
+    what = f"seed {seed} chunks {chunks} confs {one} / {two}"
+    with Batch() as b:
+        b.set_conf(DD_STEP_ONE, one)
+        b.set_conf(DD_STEP_TWO, two)
+        b.set_pipeline(chunks)
+        for plan, d_in, d_mid, d_out in zip(jobs.plans, jobs.inputs, jobs.mids, jobs.outs):
+            b.add_two_step(plan, d_in.data_ptr(), d_mid.data_ptr(), d_out.data_ptr())
+        compact = bool(b.info()["compact_intermediate"])
+        b.run(use_graph=False, stream=stream_of(torch_cuda))
+        jobs.check("batch, " + what, compact=compact)
+        jobs.reset()
+        b.run_per_job(stream=stream_of(torch_cuda))
+        jobs.check("per job, " + what)

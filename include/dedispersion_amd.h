@@ -282,6 +282,13 @@ dd_status dd_batch_add_single_step(dd_batch *batch, dd_plan *plan, const uint8_t
  * two of the synthesized beams [first_sbeam, first_sbeam + nr_sbeams). */
 dd_status dd_batch_add_two_step(dd_batch *batch, dd_plan *plan, const uint8_t *d_input, float *d_intermediate,
                                 float *d_output, uint32_t first_sbeam, uint32_t nr_sbeams);
+/* One configuration per kernel kind for all jobs.  Batch kernels are the
+ * staged (pipelined) kernels only: local = 1 and unroll <= 2 x threads_d1,
+ * and every job's row windows (the sample tile plus the largest shift spread
+ * over a DM tile) must fit their staging loads.  dd_batch_run fails with
+ * DD_ERR_CONFIGURATION when a job does not ("does not select the staged
+ * pipeline for job j" or "step-two shift spread too large for the uint16
+ * window"); fewer DMs per tile (threads_d1 x items_d1) narrow the spread. */
 dd_status dd_batch_set_conf(dd_batch *batch, dd_mode mode, const dd_conf *conf);
 /* Compact intermediate (on by default): when 255 x the live channels of
  * every subband of every two-step job is below 2^16, step one writes its

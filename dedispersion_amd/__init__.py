@@ -302,7 +302,7 @@ class Batch:
         check(self.lib.dd_batch_get_info(self.handle, ctypes.byref(i)))
         return {"jobs": i.jobs, "tiles": list(i.tiles), "workgroups": list(i.workgroups),
                 "lds_bytes": list(i.lds_bytes), "units": i.units, "compact_intermediate": i.compact_intermediate,
-                "pipeline_chunks": i.pipeline_chunks}
+                "pipeline_chunks": i.pipeline_chunks, "solo_jobs": i.solo_jobs}
 
 
 def synth_gaussian_u8(d_data: int, rows: int, cols: int, pitch: int, mean: float, std: float, seed: int,

@@ -106,6 +106,7 @@ class DDBatchInfo(ctypes.Structure):
         ("units", c_uint64),
         ("compact_intermediate", c_uint32),
         ("pipeline_chunks", c_uint32),
+        ("solo_jobs", c_uint32),
     ]
 
 

@@ -76,6 +76,11 @@ struct dd_batch {
   // one ordinary launch per job (float intermediate): dd_batch_run_per_job
   std::vector<U8Launch> pj_single, pj_one;
   std::vector<F32Launch> pj_two;
+  // jobs whose row windows do not fit the batch kernels under the batch's
+  // configuration: launched beside them with their plan's own configuration
+  // and the float intermediate
+  std::vector<U8Launch> solo_single, solo_one;
+  std::vector<F32Launch> solo_two;
   KindQueue q[3];
   DeviceBuffer<uint32_t> heads;  // claim counters, kQueues * kHeadStride per kind
   uint64_t units = 0;
@@ -134,13 +139,9 @@ std::vector<uint32_t> queue_bounds(uint32_t n) {
   return b;
 }
 
-template <typename L> dd_status check_staged(const L &launch, dd_mode mode, size_t job) {
-  if (launch.g.staged != 1)
-    return fail(DD_ERR_CONFIGURATION, "batch configuration of mode " + std::to_string((int)mode) +
-                                          " does not select the staged pipeline for job " + std::to_string(job) +
-                                          " (needs local = 1, unroll <= 2 * threads_d1 and a window that fits)");
-  return DD_OK;
-}
+// Whether a job's launch under the batch configuration is the staged
+// pipeline (its row windows fit), which the batch kernels run.
+template <typename L> bool staged(const L &launch) { return launch.g.staged == 1; }
 
 template <typename L> void merge_geometry(KindQueue &k, const L &launch, bool first) {
   if (first) {
@@ -170,6 +171,22 @@ dd_status build(dd_batch *b) {
   b->pj_single.clear();
   b->pj_one.clear();
   b->pj_two.clear();
+  b->solo_single.clear();
+  b->solo_one.clear();
+  b->solo_two.clear();
+  // the batch kernels are the staged ones: a configuration that can never
+  // select them is an error; a job whose windows do not fit runs on its own
+  for (const Job &job : b->jobs) {
+    const int m0 = job.kind == kSingleStep ? DD_SINGLE_STEP : DD_STEP_ONE;
+    const int m1 = job.kind == kSingleStep ? DD_SINGLE_STEP : DD_STEP_TWO;
+    for (int m = m0; m <= m1; ++m) {
+      const dd_conf &c = b->conf[m];
+      if (c.local != 1 || c.unroll > kStagedSlotsPerWave * c.threads_d1)
+        return fail(DD_ERR_CONFIGURATION, "batch configuration of mode " + std::to_string(m) +
+                                              " does not select the staged pipeline (needs local = 1 and unroll <= 2 "
+                                              "* threads_d1)");
+    }
+  }
   // compact intermediate: every two-step job's subband sums fit 16 bits
   b->compact = b->compact_requested;
   for (const Job &job : b->jobs)
@@ -178,29 +195,27 @@ dd_status build(dd_batch *b) {
     const Job &job = b->jobs[j];
     const dd_observation &o = job.plan->obs;
     const dd_sizes &z = job.plan->sizes;
+    const uint64_t sbeams = job.nr_sbeams ? job.nr_sbeams
+                                          : o.nr_synthesized_beams - std::min(job.first_sbeam, o.nr_synthesized_beams);
     dd_status s;
     if (job.kind == kSingleStep) {
       U8Launch L, P;
-      s = prepare_single(job.plan, b->conf[DD_SINGLE_STEP], job.in, job.out, job.first_sbeam, job.nr_sbeams, L);
-      if (s == DD_OK) s = check_staged(L, DD_SINGLE_STEP, j);
+      s = prepare_single(job.plan, job.plan->conf[DD_SINGLE_STEP], job.in, job.out, job.first_sbeam, job.nr_sbeams,
+                         P);
       if (s == DD_OK)
-        s = prepare_single(job.plan, job.plan->conf[DD_SINGLE_STEP], job.in, job.out, job.first_sbeam,
-                           job.nr_sbeams, P);
+        s = prepare_single(job.plan, b->conf[DD_SINGLE_STEP], job.in, job.out, job.first_sbeam, job.nr_sbeams, L);
       if (s != DD_OK) return s;
+      b->pj_single.push_back(P);
+      b->units += sbeams * o.nr_dms * z.nr_samples;
+      if (!staged(L)) {
+        b->solo_single.push_back(P);
+        continue;
+      }
       KindQueue &k = b->q[DD_SINGLE_STEP];
       merge_geometry(k, L, k.u8.empty());
       k.flush = k.flush || L.flush;
       k.u8.push_back(L);
-      b->pj_single.push_back(P);
-      b->units += (uint64_t)L.grid.z * o.nr_dms * z.nr_samples;
     } else {
-      U8Launch L1;
-      s = prepare_step_one(job.plan, b->conf[DD_STEP_ONE], job.in, job.mid, L1);
-      if (s == DD_OK) s = check_staged(L1, DD_STEP_ONE, j);
-      if (s != DD_OK) return s;
-      F32Launch L2;
-      s = prepare_step_two(job.plan, b->conf[DD_STEP_TWO], job.mid, job.out, job.first_sbeam, job.nr_sbeams, L2);
-      if (s != DD_OK) return s;
       // the per-job launches use each plan's own configuration
       U8Launch P1;
       F32Launch P2;
@@ -211,15 +226,31 @@ dd_status build(dd_batch *b) {
       if (s != DD_OK) return s;
       b->pj_one.push_back(P1);
       b->pj_two.push_back(P2);
-      if (b->compact) {
+      b->units += sbeams * o.nr_dms_subbanding * o.nr_dms * z.nr_samples;
+      U8Launch L1;
+      F32Launch L2;
+      s = prepare_step_one(job.plan, b->conf[DD_STEP_ONE], job.in, job.mid, L1);
+      if (s == DD_OK)
+        s = prepare_step_two(job.plan, b->conf[DD_STEP_TWO], job.mid, job.out, job.first_sbeam, job.nr_sbeams, L2);
+      if (s != DD_OK) return s;
+      bool fits = staged(L1);
+      if (fits && b->compact) {
         L1.k.out_u16 = 1;  // vector stores of four uint16 sums need 8-byte aligned rows
         L1.k.vec_store = (z.sub_out_pitch % 4 == 0) && (reinterpret_cast<uintptr_t>(job.mid) & 7) == 0;
+        // the configuration is staged and its float form prepared: what fails
+        // here is the uint16 row window or its LDS
         s = prepare_step_two_u16(job.plan, b->conf[DD_STEP_TWO], job.mid, job.out, job.first_sbeam, job.nr_sbeams,
                                  L2);
-      } else {
-        s = check_staged(L2, DD_STEP_TWO, j);
+        if (s == DD_ERR_CONFIGURATION) fits = false;
+        else if (s != DD_OK) return s;
+      } else if (fits) {
+        fits = staged(L2);
       }
-      if (s != DD_OK) return s;
+      if (!fits) {
+        b->solo_one.push_back(P1);
+        b->solo_two.push_back(P2);
+        continue;
+      }
       KindQueue &k1 = b->q[DD_STEP_ONE];
       merge_geometry(k1, L1, k1.u8.empty());
       k1.flush = k1.flush || L1.flush;
@@ -227,8 +258,6 @@ dd_status build(dd_batch *b) {
       KindQueue &k2 = b->q[DD_STEP_TWO];
       merge_geometry(k2, L2, k2.f32.empty());
       k2.f32.push_back(L2);
-      // grid.z of step two = sBeams x coarse DMs; units = sBeams x Dc x Df x S
-      b->units += (uint64_t)L2.grid.z * o.nr_dms * z.nr_samples;
     }
   }
   for (int m = 0; m < 3; ++m) {
@@ -305,8 +334,23 @@ dd_status build(dd_batch *b) {
   return DD_OK;
 }
 
-// Clears the claim counters of kind m and launches its persistent kernel.
+// The ordinary launches of kind m of the jobs that run beside the batch.
+dd_status enqueue_solo(dd_batch *b, int m, hipStream_t s) {
+  dd_status st = DD_OK;
+  if (m == DD_SINGLE_STEP)
+    for (size_t i = 0; i < b->solo_single.size() && st == DD_OK; ++i) st = run(b->solo_single[i], s);
+  else if (m == DD_STEP_ONE)
+    for (size_t i = 0; i < b->solo_one.size() && st == DD_OK; ++i) st = run(b->solo_one[i], s);
+  else
+    for (size_t i = 0; i < b->solo_two.size() && st == DD_OK; ++i) st = run(b->solo_two[i], s);
+  return st;
+}
+
+// Launches kind m of the jobs beside the batch, then clears the claim
+// counters of kind m and launches its persistent kernel.
 dd_status enqueue_kind(dd_batch *b, int m, hipStream_t s) {
+  dd_status st = enqueue_solo(b, m, s);
+  if (st != DD_OK) return st;
   KindQueue &k = b->q[m];
   if (k.n_tiles == 0) return DD_OK;
   uint32_t *heads = b->heads.ptr + m * kQueues * kHeadStride;
@@ -327,6 +371,9 @@ dd_status enqueue_pipelined(dd_batch *b, hipStream_t s) {
   DD_HIP(hipStreamWaitEvent(a, fork, 0));
   DD_HIP(hipStreamWaitEvent(c2, fork, 0));
   dd_status st = enqueue_kind(b, DD_SINGLE_STEP, a);
+  // step one of the jobs beside the batch goes first on the step-one stream,
+  // so the step-two stream's first wait covers it; their step two goes last
+  if (st == DD_OK) st = enqueue_solo(b, DD_STEP_ONE, a);
   if (st != DD_OK) return st;
   const KindQueue &k1 = b->q[DD_STEP_ONE], &k2 = b->q[DD_STEP_TWO];
   // the pipelined launches request the LDS reserves (room for the other kind)
@@ -343,6 +390,8 @@ dd_status enqueue_pipelined(dd_batch *b, hipStream_t s) {
     else if (b->n2[c])
       DD_HIP(launch_rowsum_f32_list(k2.d_f32.ptr, b->d_list2.ptr + b->off2[c], b->n2[c], g2, c2));
   }
+  st = enqueue_solo(b, DD_STEP_TWO, c2);
+  if (st != DD_OK) return st;
   DD_HIP(hipEventRecord(join_a, a));
   DD_HIP(hipEventRecord(join_b, c2));
   DD_HIP(hipStreamWaitEvent(s, join_a, 0));
@@ -499,6 +548,7 @@ dd_status dd_batch_get_info(dd_batch *batch, dd_batch_info *info) {
   info->units = batch->units;
   info->compact_intermediate = batch->compact ? 1 : 0;
   info->pipeline_chunks = (uint32_t)batch->n1.size();
+  info->solo_jobs = (uint32_t)(batch->solo_single.size() + batch->solo_one.size());
   return DD_OK;
 }
 

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define DD_ABI_VERSION 2
+#define DD_ABI_VERSION 3
 
 typedef enum dd_status {
   DD_OK = 0,
@@ -256,9 +256,10 @@ dd_status dd_tune_time(dd_plan *plan, dd_mode mode, const dd_conf *conf, const v
  * calling dd_dedisperse / dd_subband_step_one / dd_subband_step_two on each
  * job with the same configuration.  One configuration per kernel kind
  * applies to every job (DD_SINGLE_STEP, DD_STEP_ONE, DD_STEP_TWO); it must
- * select the staged pipeline (local = 1, unroll <= 2 * threads_d1, and a
- * window that fits) for every job, or dd_batch_run fails with
- * DD_ERR_CONFIGURATION.  Plans and buffers must stay alive and unchanged
+ * select the staged pipeline (local = 1, unroll <= 2 * threads_d1), or
+ * dd_batch_run fails with DD_ERR_CONFIGURATION; a job whose row windows do
+ * not fit it runs beside the batch kernels (dd_batch_set_conf).  Plans and
+ * buffers must stay alive and unchanged
  * while they are in a batch.  A batch is used by one host thread at a time. */
 typedef struct dd_batch dd_batch;
 
@@ -270,6 +271,7 @@ typedef struct dd_batch_info {
   uint64_t units;           /* output elements of the final arrays (DM trials x samples x beams) */
   uint32_t compact_intermediate; /* 1: step one writes uint16 sums (see dd_batch_set_compact_intermediate) */
   uint32_t pipeline_chunks;      /* chunks of the pipelined mode in effect, 0 = off */
+  uint32_t solo_jobs;            /* jobs run beside the batch kernels (see dd_batch_set_conf) */
 } dd_batch_info;
 
 dd_status dd_batch_create(dd_batch **batch);
@@ -283,12 +285,14 @@ dd_status dd_batch_add_single_step(dd_batch *batch, dd_plan *plan, const uint8_t
 dd_status dd_batch_add_two_step(dd_batch *batch, dd_plan *plan, const uint8_t *d_input, float *d_intermediate,
                                 float *d_output, uint32_t first_sbeam, uint32_t nr_sbeams);
 /* One configuration per kernel kind for all jobs.  Batch kernels are the
- * staged (pipelined) kernels only: local = 1 and unroll <= 2 x threads_d1,
- * and every job's row windows (the sample tile plus the largest shift spread
- * over a DM tile) must fit their staging loads.  dd_batch_run fails with
- * DD_ERR_CONFIGURATION when a job does not ("does not select the staged
- * pipeline for job j" or "step-two shift spread too large for the uint16
- * window"); fewer DMs per tile (threads_d1 x items_d1) narrow the spread. */
+ * staged (pipelined) kernels: a configuration with local != 1 or unroll > 2
+ * x threads_d1 makes dd_batch_run fail with DD_ERR_CONFIGURATION.  A job
+ * whose row windows (the sample tile plus the largest shift spread over a
+ * DM tile) do not fit the staged kernels under this configuration runs
+ * beside them, in the same run and graph, as the ordinary launches of its
+ * plan's own configuration with the float intermediate (dd_batch_info's
+ * solo_jobs counts them); fewer DMs per tile (threads_d1 x items_d1)
+ * narrow the spread. */
 dd_status dd_batch_set_conf(dd_batch *batch, dd_mode mode, const dd_conf *conf);
 /* Compact intermediate (on by default): when 255 x the live channels of
  * every subband of every two-step job is below 2^16, step one writes its

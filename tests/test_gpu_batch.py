@@ -73,14 +73,15 @@ class Jobs:
         self.torch.cuda.synchronize()
 
     def check(self, what, compact=False):
-        """compact: the intermediate buffers hold uint16 sums in their first half."""
+        """compact: the intermediate buffers hold uint16 sums in their first
+        half (a bool for every job, or one per job)."""
         self.torch.cuda.synchronize()
         for j, (w, plan) in enumerate(zip(self.ws, self.plans)):
             z = plan.sizes
             if self.mids[j] is not None:
                 want = self.want_mid[j]
                 raw = self.mids[j].cpu().numpy()
-                if compact:
+                if compact[j] if isinstance(compact, (list, tuple)) else compact:
                     raw = raw.view(np.uint16)[: raw.size].astype(np.float32)
                 got = raw.reshape(want.shape)
                 compare_rows(got, want, z.nr_samples_subbanding, f"{what} job {j} step one")
@@ -312,3 +313,50 @@ def test_batch_uint16_sums_at_the_16_bit_bound(torch_cuda):
         assert np.all(mid[:, :, : z.nr_samples_subbanding] == (255 * live)[None, :, None]), conf
         out = d_out.cpu().numpy().reshape(-1, z.out_pitch)[:, : w.obs.nr_samples_per_batch]
         assert np.all(out == 255.0 * live.sum()), conf
+
+
+def test_batch_runs_wide_jobs_beside_the_kernels(torch_cuda):
+    """A job whose row windows do not fit the staged kernels (300 MHz at the
+    bottom of the band, wide DM steps) runs beside the batch kernels with its
+    plan's own configuration and the float intermediate, in every run mode;
+    the other jobs stay in the batch with the uint16 intermediate."""
+    ws = tiny_two_step_workloads()[:2] + [
+        workloads.tiny(channels=64, samples=1000, dms=4, step=10.0, subbands=4, coarse=4, coarse_step=40.0,
+                       min_freq=300.0, bandwidth=2.0, zapped=(5,), seed=13)]
+    jobs = Jobs(torch_cuda, ws)
+    compact = [True, True, False]
+    with Batch() as b:
+        for plan, d_in, d_mid, d_out in zip(jobs.plans, jobs.inputs, jobs.mids, jobs.outs):
+            b.add_two_step(plan, d_in.data_ptr(), d_mid.data_ptr(), d_out.data_ptr())
+        for chunks in (0, 2):
+            b.set_pipeline(chunks)
+            info = b.info()
+            assert info["solo_jobs"] == 1 and info["compact_intermediate"] == 1
+            want_units = sum(w.obs.nr_synthesized_beams * w.obs.nr_dms_subbanding * w.obs.nr_dms *
+                             w.obs.nr_samples_per_batch for w in ws)
+            assert info["units"] == want_units
+            for use_graph in (False, True):
+                b.run(use_graph=use_graph, stream=stream_of(torch_cuda))
+                jobs.check(f"pipeline {chunks} graph {use_graph}", compact=compact)
+                jobs.reset()
+        b.run_mode(DD_STEP_ONE, stream=stream_of(torch_cuda))
+        b.run_mode(DD_STEP_TWO, stream=stream_of(torch_cuda))
+        jobs.check("per kind", compact=compact)
+
+
+def test_batch_runs_wide_single_step_jobs_beside_the_kernels(torch_cuda):
+    ws = [workloads.tiny(channels=64, samples=700, dms=12, step=0.9, min_freq=1300.0, bandwidth=1.0, seed=41),
+          workloads.tiny(channels=64, samples=2000, dms=12, step=20.0, min_freq=300.0, bandwidth=2.0, zapped=(5,),
+                         seed=12)]
+    jobs = Jobs(torch_cuda, ws, two_step=False)
+    with Batch() as b:
+        b.set_conf(DD_SINGLE_STEP, STAGED_SMALL)
+        for plan, d_in, d_out in zip(jobs.plans, jobs.inputs, jobs.outs):
+            b.add_single_step(plan, d_in.data_ptr(), d_out.data_ptr())
+        assert b.info()["solo_jobs"] == 1
+        for use_graph in (False, True):
+            b.run(use_graph=use_graph, stream=stream_of(torch_cuda))
+            jobs.check(f"graph {use_graph}")
+            jobs.reset()
+        b.run_mode(DD_SINGLE_STEP, stream=stream_of(torch_cuda))
+        jobs.check("per kind")

@@ -105,8 +105,9 @@ def test_two_step_random_shape(torch_cuda, seed):
 def test_batch_random_jobs(torch_cuda, seed):
     """3 to 6 random two-step jobs in one batch, uint16 intermediate, a random
     pipeline depth and staged configurations, against the oracle and against
-    per-job launches.  Jobs too wide for the step-two staging window are first
-    shown to be refused, then left out."""
+    per-job launches, persistent or pipelined, with and without a graph.
+    Jobs too wide for the uint16 step-two window run beside the batch
+    kernels."""
     rng = SplitMix64(5000 + seed)
     ws = [workloads.tiny(**two_step_shape(100 * seed + j)[0]) for j in range(3 + rng.below(4))]
     chunks = (0, 2, 3)[rng.below(3)]
@@ -118,18 +119,6 @@ def test_batch_random_jobs(torch_cuda, seed):
 
     one, two = staged_conf(), staged_conf()
     fits = [u16_window_fits(w, two) for w in ws]
-    if not all(fits):
-        # a job whose step-two rows do not fit the staging window is refused
-        # with a configuration error, not run
-        jobs = Jobs(torch_cuda, ws)
-        with Batch() as b:
-            b.set_conf(DD_STEP_ONE, one)
-            b.set_conf(DD_STEP_TWO, two)
-            for plan, d_in, d_mid, d_out in zip(jobs.plans, jobs.inputs, jobs.mids, jobs.outs):
-                b.add_two_step(plan, d_in.data_ptr(), d_mid.data_ptr(), d_out.data_ptr())
-            with pytest.raises(DDError, match="shift spread too large for the uint16 window"):
-                b.run(use_graph=False, stream=stream_of(torch_cuda))
-        ws = [w for w, f in zip(ws, fits) if f]
     jobs = Jobs(torch_cuda, ws)
     what = f"seed {seed} chunks {chunks} confs {one} / {two}"
     with Batch() as b:
@@ -138,9 +127,14 @@ def test_batch_random_jobs(torch_cuda, seed):
         b.set_pipeline(chunks)
         for plan, d_in, d_mid, d_out in zip(jobs.plans, jobs.inputs, jobs.mids, jobs.outs):
             b.add_two_step(plan, d_in.data_ptr(), d_mid.data_ptr(), d_out.data_ptr())
-        compact = bool(b.info()["compact_intermediate"])
-        b.run(use_graph=False, stream=stream_of(torch_cuda))
-        jobs.check("batch, " + what, compact=compact)
-        jobs.reset()
+        info = b.info()
+        assert info["compact_intermediate"] == 1
+        # a job too wide for the uint16 step-two window runs beside the batch
+        # kernels with its plan's own configuration and the float intermediate
+        assert info["solo_jobs"] == fits.count(False), what
+        for use_graph in (False, True):
+            b.run(use_graph=use_graph, stream=stream_of(torch_cuda))
+            jobs.check(f"batch (graph {use_graph}), " + what, compact=fits)
+            jobs.reset()
         b.run_per_job(stream=stream_of(torch_cuda))
         jobs.check("per job, " + what)

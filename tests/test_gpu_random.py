@@ -1,17 +1,19 @@
 """Seeded random shapes on the device against the oracle, bit for bit: the
-single step, the two-step path and batches of uneven two-step jobs, each
-over shapes drawn from a splitmix64 stream (channels, samples, DMs, DM steps,
-band, zapped channels, beams and their mapping, padding) and a configuration
-drawn from the valid ones.  The fixed-shape tests sweep configurations; these
-sweep shapes."""
+single step over 8-bit and packed rows, the two-step path and batches of
+uneven two-step jobs, each over shapes drawn from a splitmix64 stream
+(channels, samples, DMs, DM steps, band, zapped channels, beams and their
+mapping, padding) and a configuration drawn from the valid ones.  The
+fixed-shape tests sweep configurations; these sweep shapes."""
 import numpy as np
 import pytest
 
 import oracle
-from dedispersion_amd import DD_STEP_ONE, DD_STEP_TWO, Batch, Conf, DDError, workloads
+from dedispersion_amd import DD_SINGLE_STEP, DD_STEP_ONE, DD_STEP_TWO, Batch, Conf, Plan, workloads
 from dedispersion_amd.workloads import SplitMix64
-from tests.gpu_helpers import compare_rows, single_step, stream_of
+from tests.conftest import host_threads
+from tests.gpu_helpers import compare_rows, single_step, stream_of, zapped_flags
 from tests.test_gpu_batch import Jobs
+from tests.test_gpu_packed import packed_input
 from tests.test_gpu_parity import STEP_ONE_CONFS, STEP_TWO_CONFS, SWEEP_CONFS, two_step_case
 
 pytestmark = pytest.mark.gpu
@@ -83,6 +85,38 @@ def test_single_step_random_shape(torch_cuda, seed):
     got = out.reshape(want.shape)
     compare_rows(got, want, plan.sizes.nr_samples, f"seed {seed} {shape} padding {padding} conf {conf}")
     assert np.all(got[..., w.obs.nr_samples_per_batch:] == -1.0)
+    plan.close()
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_single_step_packed_random_shape(torch_cuda, seed):
+    """Packed rows (1, 2 or 4 bits per sample, random over the whole field)
+    over random shapes whose shift spreads fit the pipelined kernel's row
+    window, which every packed launch uses."""
+    rng = SplitMix64(3000 + seed)
+    channels = 2 + rng.below(399)
+    beams = 1 + rng.below(2)
+    w = workloads.tiny(channels=channels, samples=1 + rng.below(3000), dms=1 + rng.below(40),
+                       step=uniform(rng, 0.0, 0.5), min_freq=uniform(rng, 1300.0, 1500.0),
+                       bandwidth=uniform(rng, 0.2, 1.0), beams=beams, sbeams=beams + rng.below(2),
+                       zapped=zapped_of(rng, channels), seed=seed)
+    bits = (1, 2, 4)[rng.below(3)]
+    padding = PADDINGS[rng.below(3)]
+    conf = pick(SWEEP_CONFS, rng.below(len(SWEEP_CONFS) + 1))
+    plan = Plan(w.obs, padding, input_bits=bits)
+    orc = oracle.Oracle(w.obs, padding, threads=host_threads(), input_bits=bits)
+    z = plan.sizes
+    flags = zapped_flags(plan, w.zapped)
+    plan.set_zapped_channels(flags)
+    if conf is not None:
+        plan.set_conf(DD_SINGLE_STEP, conf)
+    host, d_in = packed_input(torch_cuda, orc, bits, z.nr_dispersed_samples, z.in_pitch, seed=seed)
+    d_out = torch_cuda.full((z.output_elements,), -1.0, dtype=torch_cuda.float32, device="cuda")
+    plan.dedisperse(d_in.data_ptr(), d_out.data_ptr(), stream=stream_of(torch_cuda))
+    torch_cuda.cuda.synchronize()
+    want = orc.dedispersion(host, flags)
+    compare_rows(d_out.cpu().numpy().reshape(want.shape), want, z.nr_samples,
+                 f"seed {seed} {bits}-bit {w.name} padding {padding} conf {conf}")
     plan.close()
 
 

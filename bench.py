@@ -161,54 +161,100 @@ def repeat_for(fn, seconds: float, max_passes: int = 100):
     return passes, dt
 
 
-def cpu_baseline(w, mode: str, seconds: float, bits: int = 8):
+def host_cpu() -> dict:
+    """The host's CPU: model (/proc/cpuinfo), logical CPUs, the CPUs this
+    process may run on, and the threads the baseline uses: all of those, or
+    the CPU share the environment grants one process (OMP_NUM_THREADS) when
+    that is smaller."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0))
+    try:
+        share = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        share = 0
+    threads = max(1, min(affinity, share) if share > 0 else affinity)
+    return {"model": model, "logical_cpus": os.cpu_count(), "affinity_cpus": affinity,
+            "omp_num_threads": share or None, "threads": threads}
+
+
+def cpu_baseline(ws, mode: str, seconds: float, bits: int = 8):
     """The oracle on the host cores over a bounded sample of the workload,
     and, beside it, on one thread (as the reference's CPU functions run)
-    over a shorter sample."""
+    over a shorter sample.  For the two-step mix the sample takes the same
+    fraction of the coarse DMs of every beam (step one over all subbands,
+    then step two over all fine DMs), so it has the mix's shapes."""
     import numpy as np
 
     import oracle
 
-    o = w.obs
     rng = np.random.default_rng(0)
-    flags = np.zeros(oracle.pad(o.nr_channels, 32), dtype=np.uint32)
-    flags[w.zapped] = 1
+    inputs = {}
+
+    def beam_input(w, orc):
+        if w.name not in inputs:
+            o = w.obs
+            pitch = orc.sub_in_pitch if mode == "two-step" else orc.in_pitch
+            inputs.clear()  # one beam's input at a time
+            inputs[w.name] = rng.integers(0, 256, size=o.nr_beams * o.nr_channels * pitch, dtype=np.uint8)
+        return inputs[w.name]
+
+    def flags_of(w):
+        f = np.zeros(oracle.pad(w.obs.nr_channels, 32), dtype=np.uint32)
+        f[w.zapped] = 1
+        return f
 
     def timed(threads: int, budget: float):
-        orc = oracle.Oracle(w.obs, 128, threads=threads, input_bits=bits)
         if mode == "two-step":
-            inp = rng.integers(0, 256, size=o.nr_beams * o.nr_channels * orc.sub_in_pitch, dtype=np.uint8)
-            # calibrate on one coarse DM, then size the sample to ~budget
-            t = time.perf_counter()
-            mid = orc.step_one(inp, flags, rows=(0, 1))
-            orc.step_two(mid.ravel(), rows=(0, o.nr_dms))
-            per = time.perf_counter() - t
-            n = max(1, min(o.nr_dms_subbanding, int(budget / max(per, 1e-3))))
-
-            def one_pass():
+            def run(w, n):
+                o = w.obs
+                orc = oracle.Oracle(o, 128, threads=threads, input_bits=bits)
+                inp, flags = beam_input(w, orc), flags_of(w)
+                t = time.perf_counter()
                 mid = orc.step_one(inp, flags, rows=(0, n))
                 orc.step_two(mid.ravel(), rows=(0, n * o.nr_dms))
+                return time.perf_counter() - t, n * o.nr_dms * (o.nr_samples_per_batch // o.downsampling)
 
-            passes, dt = repeat_for(one_pass, budget)
-            units = passes * n * o.nr_dms * (o.nr_samples_per_batch // o.downsampling)
-            sample = (f"{passes} pass(es) over {n} of {o.nr_dms_subbanding} coarse DMs (step one, all subbands, "
-                      f"then step two, {o.nr_dms} fine DMs each) of {w.name}")
-        else:
-            inp = rng.integers(0, 256, size=o.nr_beams * o.nr_channels * orc.in_pitch, dtype=np.uint8)
-            t = time.perf_counter()
-            orc.dedispersion(inp, flags, rows=(0, threads))
-            per = (time.perf_counter() - t) / threads
-            n = max(threads, min(o.nr_dms, int(budget / max(per, 1e-4) * threads)))
-            passes, dt = repeat_for(lambda: orc.dedispersion(inp, flags, rows=(0, n)), budget)
-            units = passes * n * (o.nr_samples_per_batch // o.downsampling)
-            sample = f"{passes} pass(es) over {n} of {o.nr_dms} DM rows of {w.name}"
-        return units / dt, sample, dt
+            # calibrate on one coarse DM of the first beam, then take the same
+            # fraction of every beam's coarse DMs
+            dt, units = run(ws[0], 1)
+            rate = units / max(dt, 1e-4)
+            total = sum(w.obs.nr_dms_subbanding * w.obs.nr_dms * w.obs.nr_samples_per_batch for w in ws)
+            frac = min(1.0, budget * rate / total)
+            spent, done = 0.0, 0
+            for w in ws:
+                n = max(1, round(frac * w.obs.nr_dms_subbanding))
+                dt, units = run(w, n)
+                spent += dt
+                done += units
+            sample = (f"{frac:.3f} of the coarse DMs of each of the {len(ws)} beams of the mix (at least one; step "
+                      f"one over all subbands, then step two over all fine DMs)")
+            return done / spent, sample, spent
+        w = ws[0]
+        o = w.obs
+        orc = oracle.Oracle(o, 128, threads=threads, input_bits=bits)
+        inp, flags = beam_input(w, orc), flags_of(w)
+        t = time.perf_counter()
+        orc.dedispersion(inp, flags, rows=(0, threads))
+        per = (time.perf_counter() - t) / threads
+        n = max(threads, min(o.nr_dms, int(budget / max(per, 1e-4) * threads)))
+        passes, dt = repeat_for(lambda: orc.dedispersion(inp, flags, rows=(0, n)), budget)
+        units = passes * n * (o.nr_samples_per_batch // o.downsampling)
+        return units / dt, f"{passes} pass(es) over {n} of {o.nr_dms} DM rows of {w.name}", dt
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    host = host_cpu()
+    threads = host["threads"]
     value, sample, dt = timed(threads, seconds)
     value1, sample1, dt1 = timed(1, seconds / 3)
     return {"value": value, "unit": "DM-trial*samples/s", "cores": threads, "kind": "port", "sample": sample,
-            "seconds": round(dt, 3),
+            "seconds": round(dt, 3), "host": host,
             "single_thread": {"value": value1, "cores": 1, "sample": sample1, "seconds": round(dt1, 3)}}
 
 
@@ -330,7 +376,7 @@ def run_batched(args, torch):
                      "algorithmic_bytes": step_bytes[dominant]},
         "wall_s": wall,
     }
-    return line, ws[0], "two-step"
+    return line, ws, "two-step"
 
 
 def run_detect(args, torch):
@@ -473,7 +519,8 @@ def main():
         line, w, mode = run_single_plan(args, torch)
     if not args.no_cpu_baseline and w is not None:
         log("cpu baseline ...")
-        line["cpu_baseline"] = cpu_baseline(w, mode, args.cpu_seconds, args.input_bits)
+        line["cpu_baseline"] = cpu_baseline(w if isinstance(w, list) else [w], mode, args.cpu_seconds,
+                                            args.input_bits)
     print(json.dumps(line), flush=True)
 
 

@@ -107,7 +107,8 @@ class Plan:
     """A dd_plan: observation, padding, device-resident tables, launches."""
 
     def __init__(self, obs: Observation, padding: int = 128, input_bits: int = 8):
-        """input_bits: the reference's inputBits (1, 2, 4 = packed input rows, 8)."""
+        """input_bits: the reference's inputBits (1, 2, 4 = packed input rows,
+        8, or 32 = float input rows for dedisperse_f32 / step_one_f32)."""
         self.lib = load()
         self.obs = obs
         self.padding = padding
@@ -159,6 +160,15 @@ class Plan:
         conf = tuned[device][nr_dms]
         self.set_conf(mode, conf)
         return conf
+
+    def set_step_one_shifts(self, absolute: bool) -> None:
+        """Step one's delay per channel: relative to its subband's top channel
+        (the reference's CPU function, the default) or its own (the
+        reference's OpenCL kernel)."""
+        check(self.lib.dd_plan_set_step_one_shifts(self.handle, 1 if absolute else 0))
+        s = _lib.DDSizes()
+        check(self.lib.dd_plan_get_sizes(self.handle, ctypes.byref(s)))
+        self.sizes = s
 
     def get_conf(self, mode: int) -> Conf:
         c = _lib.DDConf()
@@ -217,6 +227,14 @@ class Plan:
     def step_one(self, d_input: int, d_intermediate: int, stream: int = 0):
         check(self.lib.dd_subband_step_one(self.handle, d_input, d_intermediate, stream or None))
 
+    def dedisperse_f32(self, d_input: int, d_output: int, first_sbeam: int = 0, nr_sbeams: int = 0, stream: int = 0):
+        """The single step over float input rows (a plan with input_bits 32)."""
+        check(self.lib.dd_dedisperse_f32(self.handle, d_input, d_output, first_sbeam, nr_sbeams, stream or None))
+
+    def step_one_f32(self, d_input: int, d_intermediate: int, stream: int = 0):
+        """Step one over float input rows (a plan with input_bits 32)."""
+        check(self.lib.dd_subband_step_one_f32(self.handle, d_input, d_intermediate, stream or None))
+
     def step_two(self, d_intermediate: int, d_output: int, first_sbeam: int = 0, nr_sbeams: int = 0, stream: int = 0):
         check(self.lib.dd_subband_step_two(self.handle, d_intermediate, d_output, first_sbeam, nr_sbeams, stream or None))
 
@@ -273,6 +291,15 @@ class Batch:
     def set_conf(self, mode: int, conf: Conf) -> None:
         c = conf.to_c()
         check(self.lib.dd_batch_set_conf(self.handle, mode, ctypes.byref(c)))
+
+    def set_step_one_shifts(self, absolute: bool) -> None:
+        """Step one's delay per channel: relative to its subband's top channel
+        (the reference's CPU function, the default) or its own (the
+        reference's OpenCL kernel)."""
+        check(self.lib.dd_plan_set_step_one_shifts(self.handle, 1 if absolute else 0))
+        s = _lib.DDSizes()
+        check(self.lib.dd_plan_get_sizes(self.handle, ctypes.byref(s)))
+        self.sizes = s
 
     def get_conf(self, mode: int) -> Conf:
         c = _lib.DDConf()

@@ -84,6 +84,9 @@ class DDSizes(ctypes.Structure):
         ("max_shift_step_one", c_uint32),
         ("max_shift_step_two", c_uint32),
         ("input_bits", c_uint32),
+        ("nr_samples_step_one", c_uint32),
+        ("step_two_in_pitch", c_uint64),
+        ("step_two_input_elements", c_uint64),
     ]
 
 
@@ -156,6 +159,9 @@ PROTOTYPES = {
     "dd_plan_get_integer_shifts": (c_int, [c_void_p, c_int, POINTER(c_uint32)]),
     "dd_dedisperse": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_void_p]),
     "dd_subband_step_one": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "dd_dedisperse_f32": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_void_p]),
+    "dd_plan_set_step_one_shifts": (c_int, [c_void_p, c_int]),
+    "dd_subband_step_one_f32": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "dd_subband_step_two": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_void_p]),
     "dd_detect_peaks": (c_int, [c_void_p, c_uint64, c_uint32, c_uint64, c_uint32, c_void_p, c_void_p]),
     "dd_tune_time": (

@@ -409,6 +409,11 @@ dd_status enqueue(dd_batch *b, hipStream_t s) {
 }
 
 dd_status add_job(dd_batch *b, const Job &job) {
+  if (job.plan->input_bits == 32)
+    return fail(DD_ERR_UNSUPPORTED, "batches take 8-bit and packed input; launch float-input plans on their own");
+  if (job.kind == kTwoStep && job.plan->obs.downsampling != 1)
+    return fail(DD_ERR_UNSUPPORTED, "a batch chains step one into step two, whose intermediate pitches differ at "
+                                    "downsampling > 1; launch the two steps of such a plan on their own");
   b->jobs.push_back(job);
   b->invalidate();
   return DD_OK;

@@ -77,9 +77,14 @@ constexpr uint32_t kFactoredPatternWord = 4;
 constexpr uint32_t kFactoredDMWord = kFactoredPatternWord + kFactoredMaxPatterns;
 constexpr uint32_t kFactoredRecordDwords = kFactoredDMWord + kFactoredDMs / 2;
 
-// Shifted row sum over float rows: step two (include/Dedispersion.hpp:155-176).
-// Output element (sbeam, coarse, dm, t) = sum over subbands sb = 0..Nsb-1, in
-// order, of in[inbeam(sbeam, sb)][coarse][sb][t + ishift[sb][dm]].
+// Shifted row sum over float rows: step two (include/Dedispersion.hpp:155-176),
+// and the single step and step one over float input (:70-153 with I = float).
+// A launch's slices are z = outer * n_coarse + inner.  Output element
+// (outer, inner, dm, t) = the float sum, in order, over the rows r of the
+// slice of in[inbeam(outer, r)][inner][r][t + ishift[r][dm]], starting from 0.
+// Step two: outer = sBeam, inner = coarse DM, rows = subbands 0..Nsb-1.
+// Float single step: outer = sBeam, one inner, rows = the live channels.
+// Float step one: outer = beam, inner = subband, rows = its live channels.
 struct RowSumF32Params {
   const float *in;
   uint64_t in_floats;         // floats addressable from `in`
@@ -89,13 +94,13 @@ struct RowSumF32Params {
   const uint32_t *map;        // [sBeam][map_pitch]
   uint32_t map_pitch;
   uint32_t first_sbeam;
-  uint32_t n_rows;            // subbands
-  uint32_t n_coarse;          // grid z = sBeams * n_coarse
-  uint64_t in_row_pitch;      // floats between subband rows
-  uint64_t in_coarse_pitch;   // floats between coarse DMs
-  uint64_t in_beam_pitch;     // floats between beams
+  uint32_t n_rows;            // rows of every slice without a row list (subbands)
+  uint32_t n_coarse;          // inner indices per outer index: grid z = outer * n_coarse
+  uint64_t in_row_pitch;      // floats between rows
+  uint64_t in_coarse_pitch;   // floats between inner indices (coarse DMs)
+  uint64_t in_beam_pitch;     // floats between input beams
   uint64_t out_dm_pitch;      // floats between output DM rows
-  uint64_t out_beam_pitch;
+  uint64_t out_beam_pitch;    // floats between outer indices
   uint32_t n_samples;
   uint32_t n_dms;             // fine DMs
   uint32_t unroll;
@@ -111,6 +116,15 @@ struct RowSumF32Params {
   // (1, 2 or 4) may be added as 16-bit pairs before widening, which needs
   // combine_rows x 255 x (live channels of any subband) <= 65535.
   uint32_t combine_rows;
+  // Row lists (float and f32 kernels only): when `rows` is set, slice
+  // (outer, inner) sums rows[inner * rows_pitch + k] for k < rows_count[inner],
+  // in that order (each < 65536); otherwise rows 0 .. n_rows - 1.  A row's
+  // input beam is map[(first_sbeam + outer) * map_pitch + row] when map is
+  // set, else outer (each < 65536).
+  const uint32_t *rows;
+  const uint32_t *rows_count;
+  uint32_t rows_pitch;        // also the most rows of a slice
+  uint64_t out_inner_pitch;   // floats between the inner indices of one outer index
 };
 
 // Geometry of one launch.
@@ -201,7 +215,9 @@ struct ShiftFormula {
   uint32_t padded_rows;        // table length
 };
 hipError_t launch_build_shifts(const ShiftFormula &f, float *d_table, hipStream_t stream);
-hipError_t launch_relative_shifts(const float *d_table, uint32_t channels, uint32_t per_subband,
+// Step one's delay table: each channel's delay minus that of its subband's
+// top channel, or (absolute) the channel's own delay.
+hipError_t launch_relative_shifts(const float *d_table, uint32_t channels, uint32_t per_subband, bool absolute,
                                   float *d_relative, hipStream_t stream);
 hipError_t launch_integer_shifts(const float *d_table, uint32_t rows, uint32_t dms, uint32_t dm_pitch,
                                  float first_dm, float dm_step, uint32_t *d_out, hipStream_t stream);

@@ -1,11 +1,14 @@
-// kernels_f32.hip -- step two of the subband path on CDNA4 (gfx950).
+// kernels_f32.hip -- shifted row sums over float rows on CDNA4 (gfx950):
+// step two of the subband path, and the single step and step one over float
+// input.
 //
-// Reference: CPU loop include/Dedispersion.hpp:155-176, OpenCL generator
-// :884-1122.  A workgroup owns a (fine-DM tile x sample tile) block of one
-// (synthesized beam, coarse DM) pair and walks the subbands in order 0..Nsb-1,
-// adding into float accumulators.  The order matches the reference's
-// sequential float sum, so the result is bit-identical for any float input,
-// not only for the integer-valued output of step one.
+// Reference: CPU loops include/Dedispersion.hpp:155-176 (and :70-153 with
+// I = float), OpenCL generator :884-1122.  A workgroup owns a (DM tile x
+// sample tile) block of one slice -- (synthesized beam, coarse DM) for step
+// two -- and walks the slice's rows in order (subbands 0..Nsb-1, or the live
+// channels), adding into float accumulators.  The order matches the
+// reference's sequential float sum, so the result is bit-identical for any
+// float input, not only for the integer-valued output of step one.
 //
 // Each wavefront owns `DPW` consecutive fine DMs (wave-uniform shifts);
 // lane l owns samples t0 + l + 64k, k < RUN, so LDS reads and global stores
@@ -42,9 +45,11 @@ __global__ void __launch_bounds__(512) rowsum_f32(const RowSumF32Params p) {
   const uint32_t t0 = blockIdx.x * TS;
   const uint32_t d0 = blockIdx.y * nwaves * DPW;
   const uint32_t dw = d0 + wave * DPW;
-  const uint32_t coarse = blockIdx.z % p.n_coarse;
-  const uint32_t sbeam = blockIdx.z / p.n_coarse;
+  const uint32_t inner = blockIdx.z % p.n_coarse;
+  const uint32_t outer = blockIdx.z / p.n_coarse;
   const uint32_t W = p.win_floats;
+  const uint32_t *list = p.rows ? p.rows + (uint64_t)inner * p.rows_pitch : nullptr;
+  const uint32_t nrows = list ? uniform(p.rows_count[inner]) : p.n_rows;
 
   float acc[DPW][RUN];
 #pragma unroll
@@ -52,18 +57,20 @@ __global__ void __launch_bounds__(512) rowsum_f32(const RowSumF32Params p) {
 #pragma unroll
     for (int k = 0; k < RUN; ++k) acc[j][k] = 0.0f;
 
+  // the row of list entry k, and the input offset of a row's window start
+  auto row_of = [&](uint32_t k) -> uint32_t { return list ? uniform(list[k]) : k; };
   auto row_offset = [&](uint32_t sb) -> uint64_t {
-    const uint32_t in_beam = p.map[(uint64_t)(p.first_sbeam + sbeam) * p.map_pitch + sb];
-    return (uint64_t)in_beam * p.in_beam_pitch + (uint64_t)coarse * p.in_coarse_pitch +
+    const uint32_t in_beam = p.map ? p.map[(uint64_t)(p.first_sbeam + outer) * p.map_pitch + sb] : outer;
+    return (uint64_t)in_beam * p.in_beam_pitch + (uint64_t)inner * p.in_coarse_pitch +
            (uint64_t)sb * p.in_row_pitch + t0;
   };
 
   const uint32_t G = p.unroll;
-  for (uint32_t s0 = 0; s0 < p.n_rows; s0 += G) {
-    const uint32_t n = min(G, p.n_rows - s0);
+  for (uint32_t s0 = 0; s0 < nrows; s0 += G) {
+    const uint32_t n = min(G, nrows - s0);
     if constexpr (LDS) {
       for (uint32_t g = wave; g < n; g += nwaves) {
-        const uint32_t sb = s0 + g;
+        const uint32_t sb = row_of(s0 + g);
         const uint32_t shmin = p.ishift[(uint64_t)sb * p.dm_pitch + d0];
         const uint64_t off = row_offset(sb) + shmin;
         const uint32_t recs = records_from(p.in_floats, off);
@@ -83,7 +90,7 @@ __global__ void __launch_bounds__(512) rowsum_f32(const RowSumF32Params p) {
       __syncthreads();
     }
     for (uint32_t g = 0; g < n; ++g) {
-      const uint32_t sb = s0 + g;
+      const uint32_t sb = row_of(s0 + g);
       const uint32_t *shrow = p.ishift + (uint64_t)sb * p.dm_pitch;
       if constexpr (LDS) {
         const uint32_t shmin = shrow[d0];
@@ -113,8 +120,8 @@ __global__ void __launch_bounds__(512) rowsum_f32(const RowSumF32Params p) {
   for (int j = 0; j < DPW; ++j) {
     const uint32_t dm = dw + j;
     if (dm >= p.n_dms) break;
-    float *orow = p.out + (uint64_t)sbeam * p.out_beam_pitch +
-                  ((uint64_t)coarse * p.n_dms + dm) * p.out_dm_pitch;
+    float *orow = p.out + (uint64_t)outer * p.out_beam_pitch + (uint64_t)inner * p.out_inner_pitch +
+                  (uint64_t)dm * p.out_dm_pitch;
 #pragma unroll
     for (int k = 0; k < RUN; ++k) {
       const uint32_t t = t0 + lane + 64 * k;

@@ -1,14 +1,16 @@
-// kernels_f32_staged.hip -- pipelined step two over float rows.
+// kernels_f32_staged.hip -- pipelined shifted row sums over float rows.
 //
-// Same operation and arithmetic as kernels_f32.hip
-// (include/Dedispersion.hpp:155-176): subbands are added in order 0..Nsb-1
-// into float accumulators, bit-identical to the reference's sequential float
-// sum for any input.  The pipeline is that of kernels_u8_staged.hip:
-// double-buffered subband staging in LDS with one barrier per group, integer
-// shifts computed once per staged row from the float delay table and
-// published through LDS, the workgroup's input-beam mapping copied to LDS,
-// XCD-aware workgroup order.  Lane l owns samples t0 + l + 64k and reads a
-// DM's run with ds_read2st64_b32 pairs from one base address.
+// Same operation and arithmetic as kernels_f32.hip: step two
+// (include/Dedispersion.hpp:155-176), and the single step and step one over
+// float input (:70-153 with I = float).  A slice's rows (subbands, or live
+// channels) are added in order into float accumulators that start at 0,
+// bit-identical to the reference's sequential float sum for any input.  The
+// pipeline is that of kernels_u8_staged.hip: double-buffered row staging in
+// LDS with one barrier per group, integer shifts computed once per staged row
+// from the float delay table and published through LDS, the slice's rows and
+// input beams copied to LDS, XCD-aware workgroup order.  Lane l owns samples
+// t0 + l + 64k and reads a DM's run with ds_read2st64_b32 pairs from one
+// base address.
 #include "staged_common.hpp"
 
 namespace ddamd {
@@ -16,8 +18,8 @@ namespace {
 
 using namespace staged;
 
-// One (sBeam x coarse DM, fine-DM tile, sample tile) block: logical tile
-// index `logical` of the launch described by p (DM tile fastest).
+// One (slice, DM tile, sample tile) block: logical tile index `logical` of
+// the launch described by p (DM tile fastest).
 template <int RUN, int DPW>
 __device__ __forceinline__ void f32_tile(const RowSumF32Params &p, uint32_t logical, float *ldsf) {
   constexpr int TS = 64 * RUN;
@@ -34,22 +36,28 @@ __device__ __forceinline__ void f32_tile(const RowSumF32Params &p, uint32_t logi
   const uint32_t dw = wave * DPW;
   // a wavefront whose DMs all lie past the last DM (a partial DM tile) only stages rows
   const bool busy = d0 + dw < p.n_dms;
-  const uint32_t coarse = tile.z % p.n_coarse;
-  const uint32_t sbeam = tile.z / p.n_coarse;
+  const uint32_t inner = tile.z % p.n_coarse;
+  const uint32_t outer = tile.z / p.n_coarse;
   const uint32_t G = p.unroll;
   const uint32_t WF = p.win_floats;  // floats per window (multiple of 4)
   const uint32_t NCH = WF / 4;
   const uint32_t dmax = p.n_dms - 1;
-  const uint32_t rows_pad = (2 * p.n_rows + 3) & ~3u;
+  const uint32_t *list = p.rows ? p.rows + (uint64_t)inner * p.rows_pitch : nullptr;
+  const uint32_t nrows = list ? uniform(p.rows_count[inner]) : p.n_rows;
+  const uint32_t slots = list ? p.rows_pitch : p.n_rows;  // row entries in LDS
+  const uint32_t rows_pad = (2 * slots + 3) & ~3u;
 
-  uint32_t *lbeam = reinterpret_cast<uint32_t *>(ldsf);
-  float *ltab = ldsf + p.n_rows;
+  // LDS: rows (row | input beam << 16, float delay) | windows [2][G][WF] | DM offsets [2][G][TD]
+  uint32_t *lrow = reinterpret_cast<uint32_t *>(ldsf);
+  float *ltab = ldsf + slots;
   float *windows = ldsf + rows_pad;
   uint32_t *meta = reinterpret_cast<uint32_t *>(windows + 2 * G * WF);
 
-  for (uint32_t sb = threadIdx.x; sb < p.n_rows; sb += blockDim.x) {
-    lbeam[sb] = p.map[(uint64_t)(p.first_sbeam + sbeam) * p.map_pitch + sb];
-    ltab[sb] = p.table[sb];
+  for (uint32_t k = threadIdx.x; k < nrows; k += blockDim.x) {
+    const uint32_t row = list ? list[k] : k;
+    const uint32_t in_beam = p.map ? p.map[(uint64_t)(p.first_sbeam + outer) * p.map_pitch + row] : outer;
+    lrow[k] = row | (in_beam << 16);
+    ltab[k] = p.table[row];
   }
   __syncthreads();
 
@@ -60,14 +68,14 @@ __device__ __forceinline__ void f32_tile(const RowSumF32Params &p, uint32_t logi
     for (int s = 0; s < SPW; ++s) {
       const uint32_t g = wave + s * nwaves;
       if (g < nn) {
-        const uint32_t sb = s0 + g;
-        const uint32_t in_beam = uniform(lbeam[sb]);
-        const float tab = uniformf(ltab[sb]);
+        const uint32_t packed = uniform(lrow[s0 + g]);
+        const float tab = uniformf(ltab[s0 + g]);
+        const uint32_t row = packed & 0xFFFFu, in_beam = packed >> 16;
         const uint32_t sh = shift_of(p.first_dm, p.dm_step, min(d0 + lane, dmax), tab);
         const uint32_t shmin = uniform(sh);
         so[s] = sh - shmin;
-        const uint64_t off = uniform64((uint64_t)in_beam * p.in_beam_pitch + (uint64_t)coarse * p.in_coarse_pitch +
-                                       (uint64_t)sb * p.in_row_pitch + t0 + shmin);
+        const uint64_t off = uniform64((uint64_t)in_beam * p.in_beam_pitch + (uint64_t)inner * p.in_coarse_pitch +
+                                       (uint64_t)row * p.in_row_pitch + t0 + shmin);
         const uint32_t recs = records_from_floats(p.in_floats, off);
         const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(p.in + off, recs);
 #pragma unroll
@@ -121,16 +129,16 @@ __device__ __forceinline__ void f32_tile(const RowSumF32Params &p, uint32_t logi
   };
 
   uint32_t cur = 0;
-  {
-    const uint32_t n0 = min(G, p.n_rows);
+  if (nrows > 0) {
+    const uint32_t n0 = min(G, nrows);
     stage_load(0, n0);
     stage_store(0, n0);
   }
   __syncthreads();
-  for (uint32_t s0 = 0; s0 < p.n_rows; s0 += G) {
-    const uint32_t n = min(G, p.n_rows - s0);
+  for (uint32_t s0 = 0; s0 < nrows; s0 += G) {
+    const uint32_t n = min(G, nrows - s0);
     const uint32_t sn = s0 + G;
-    const uint32_t nn = sn < p.n_rows ? min(G, p.n_rows - sn) : 0;
+    const uint32_t nn = sn < nrows ? min(G, nrows - sn) : 0;
     if (nn) stage_load(sn, nn);
     if (busy) compute(cur, n);
     if (nn) stage_store(cur ^ 1, nn);
@@ -142,7 +150,8 @@ __device__ __forceinline__ void f32_tile(const RowSumF32Params &p, uint32_t logi
   for (int j = 0; j < DPW; ++j) {
     const uint32_t dm = d0 + dw + j;
     if (dm >= p.n_dms) continue;
-    float *orow = p.out + (uint64_t)sbeam * p.out_beam_pitch + ((uint64_t)coarse * p.n_dms + dm) * p.out_dm_pitch;
+    float *orow = p.out + (uint64_t)outer * p.out_beam_pitch + (uint64_t)inner * p.out_inner_pitch +
+                  (uint64_t)dm * p.out_dm_pitch;
 #pragma unroll
     for (int k = 0; k < RUN; ++k) {
       const uint32_t t = t0 + lane + 64 * k;

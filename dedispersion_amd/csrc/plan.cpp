@@ -50,24 +50,27 @@ float subband_max_freq(const dd_observation &o) {
   return subband_min_freq(o) + span;
 }
 
-// log2 of the samples per input byte: 0 for bytes, 1, 2, 3 for 4-, 2- and
-// 1-bit samples.
+// log2 of the samples per input byte: 0 for bytes (and float input), 1, 2, 3
+// for 4-, 2- and 1-bit samples.
 uint32_t input_log2f(const dd_plan *p) {
   return p->input_bits == 1 ? 3u : p->input_bits == 2 ? 2u : p->input_bits == 4 ? 1u : 0u;
 }
 
-// Input row pitch and sizes.  A row is pad(S_disp / (8 / inputBits), padding)
-// bytes (include/Dedispersion.hpp:90, :97, :133, :140), the division rounded up
-// so that the row holds every sample when 8 / inputBits does not divide S_disp.
+// Input row pitch and sizes, in elements of the input type.  A row is
+// pad(S_disp / (8 / inputBits), padding) bytes (include/Dedispersion.hpp:90,
+// :97, :133, :140), the division rounded up so that the row holds every
+// sample when 8 / inputBits does not divide S_disp; float rows are
+// pad(S_disp, padding / 4) floats.
 void size_inputs(dd_plan *p) {
   dd_sizes &z = p->sizes;
   const uint32_t per = 1u << input_log2f(p);
+  const uint32_t pad_elems = p->input_bits == 32 ? p->padding / 4 : p->padding;
   const uint64_t C = p->obs.nr_channels;
   z.input_bits = p->input_bits;
-  z.in_pitch = padto((z.nr_dispersed_samples + per - 1) / per, p->padding);
+  z.in_pitch = padto((z.nr_dispersed_samples + per - 1) / per, pad_elems);
   z.input_elements = (uint64_t)p->obs.nr_beams * C * z.in_pitch;
   if (p->two_step) {
-    z.sub_in_pitch = padto((z.nr_dispersed_samples_subbanding + per - 1) / per, p->padding);
+    z.sub_in_pitch = padto((z.nr_dispersed_samples_subbanding + per - 1) / per, pad_elems);
     z.sub_input_elements = (uint64_t)p->obs.nr_beams * C * z.sub_in_pitch;
   }
 }
@@ -103,7 +106,8 @@ dd_status rebuild_integer_tables(dd_plan *p) {
   if (s != DD_OK) return s;
   if (p->two_step) {
     DD_HIP(p->relative.allocate(o.nr_channels));
-    DD_HIP(launch_relative_shifts(p->shifts.ptr, o.nr_channels, cps(p), p->relative.ptr, nullptr));
+    DD_HIP(launch_relative_shifts(p->shifts.ptr, o.nr_channels, cps(p), p->step_one_absolute, p->relative.ptr,
+                                  nullptr));
     s = build_integer(p, p->ish_one, p->relative.ptr, o.nr_channels, o.nr_dms_subbanding, o.first_dm_subbanding,
                       o.dm_step_subbanding);
     if (s != DD_OK) return s;
@@ -260,6 +264,16 @@ dd_status geometry_of(const dd_plan *p, dd_mode mode, const dd_conf &c, Geometry
   g.items_d1 = c.items_d1;
   g.packed = 0;
   g.unroll = c.unroll;
+  // rows of floats: step two, and the single step and step one over float input
+  const bool float_rows = mode == DD_STEP_TWO || p->input_bits == 32;
+  if (float_rows && mode != DD_STEP_TWO && c.local == 3) {
+    // the channel-block pattern kernel sums bytes: float rows run the
+    // staged pipeline of the same shape
+    dd_conf staged = c;
+    staged.local = 1;
+    staged.unroll = std::min(c.unroll, kStagedSlotsPerWave * c.threads_d1);
+    return geometry_of(p, mode, staged, g);
+  }
   if (mode != DD_STEP_TWO && p->input_bits < 8 &&
       (c.local != 1 || c.unroll > kStagedSlotsPerWave * c.threads_d1)) {
     // packed rows are unpacked as they are staged: only the pipelined kernel
@@ -313,16 +327,27 @@ dd_status geometry_of(const dd_plan *p, dd_mode mode, const dd_conf &c, Geometry
   g.staged = 0;
   const uint32_t ts = 64 * c.items_d0;
   const bool stageable = c.local == 1 && c.unroll <= kStagedSlotsPerWave * c.threads_d1;
-  if (mode == DD_STEP_TWO) {
+  if (float_rows) {
     if (!rowsum_f32_supported(c.items_d0, c.items_d1))
       return fail(DD_ERR_CONFIGURATION, "items_d0 must be 4, 8 or 16 and items_d1 1, 2, 4 or 8");
     if (g.local) {
       // a lane reads floats [spread + lane + 64k] for k < items_d0
-      g.win = roundup(ts + p->ish_two.max_spread(tile), 4);
+      const ShiftTable &t = mode == DD_SINGLE_STEP ? p->ish_single : mode == DD_STEP_ONE ? p->ish_one : p->ish_two;
+      g.win = roundup(ts + t.max_spread(tile), 4);
       g.lds_bytes = c.unroll * g.win * 4;
       if (stageable && g.win / 4 <= 64 * f32_staged_chunks(c.items_d0)) {
-        g.staged = 1;
-        g.lds_bytes = 4 * (roundup(2 * p->obs.nr_subbands, 4) + 2 * c.unroll * g.win + 2 * c.unroll * tile);
+        // rows (row | beam, delay) of a slice | [2 buffers][unroll rows][win] | DM offsets;
+        // rows staged per barrier are capped at what fits in LDS
+        const uint32_t slots = mode == DD_STEP_TWO ? p->obs.nr_subbands
+                               : mode == DD_SINGLE_STEP ? p->obs.nr_channels : cps(p);
+        auto lds_of = [&](uint32_t u) { return 4 * (roundup(2 * slots, 4) + 2 * u * g.win + 2 * u * tile); };
+        uint32_t unroll = c.unroll;
+        while (unroll > 1 && lds_of(unroll) > p->max_lds) --unroll;
+        if (lds_of(unroll) <= p->max_lds) {
+          g.staged = 1;
+          g.unroll = unroll;
+          g.lds_bytes = lds_of(unroll);
+        }
       }
     }
   } else {
@@ -343,10 +368,15 @@ dd_status geometry_of(const dd_plan *p, dd_mode mode, const dd_conf &c, Geometry
         auto lds_of = [&](uint32_t u) { return 4 * (roundup(2 * live_pitch, 4) + 2 * u * 4 * (g.win / 4) + 2 * u * tile); };
         uint32_t unroll = c.unroll;
         while (unroll > 1 && lds_of(unroll) > p->max_lds) --unroll;
-        g.staged = 1;
-        g.packed = per > 1;
-        g.unroll = unroll;
-        g.lds_bytes = lds_of(unroll);
+        if (lds_of(unroll) > p->max_lds && per == 1) {
+          // not even one row per barrier fits beside the row table: the
+          // one-group staging (or the direct kernel) below
+        } else {
+          g.staged = 1;
+          g.packed = per > 1;
+          g.unroll = unroll;
+          g.lds_bytes = lds_of(unroll);
+        }
       } else if (per > 1) {
         return fail(DD_ERR_CONFIGURATION, "packed input needs the pipelined kernel, whose row window is at most " +
                                               std::to_string(16 * 64 * kU8StagedChunks) + " bytes; this one needs " +
@@ -374,12 +404,13 @@ bool aligned16(const void *ptr) { return (reinterpret_cast<uintptr_t>(ptr) & 15)
 dd_status prepare_single(dd_plan *plan, const dd_conf &c, const uint8_t *d_input, float *d_output,
                          uint32_t first_sbeam, uint32_t nr_sbeams, U8Launch &L) {
   if (!plan || !d_input || !d_output) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  if (plan->input_bits == 32) return fail(DD_ERR_INVALID_ARGUMENT, "the plan takes float input (dd_dedisperse_f32)");
   const dd_observation &o = plan->obs;
   const dd_sizes &z = plan->sizes;
   if (nr_sbeams == 0) nr_sbeams = o.nr_synthesized_beams - std::min(first_sbeam, o.nr_synthesized_beams);
   if (nr_sbeams == 0 || first_sbeam + nr_sbeams > o.nr_synthesized_beams)
     return fail(DD_ERR_INVALID_ARGUMENT, "synthesized beam range out of bounds");
-  if ((uint64_t)z.nr_samples + z.max_shift > (z.in_pitch << input_log2f(plan)))
+  if ((uint64_t)z.nr_samples + z.max_shift > z.nr_dispersed_samples)
     return fail(DD_ERR_INVALID_ARGUMENT, "dispersed batch shorter than samples + largest shift");
   dd_status s = geometry_of(plan, DD_SINGLE_STEP, c, L.g);
   if (s != DD_OK) return s;
@@ -431,9 +462,11 @@ dd_status prepare_step_one(dd_plan *plan, const dd_conf &c, const uint8_t *d_inp
                            U8Launch &L) {
   if (!plan || !d_input || !d_intermediate) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
   if (!plan->two_step) return fail(DD_ERR_INVALID_ARGUMENT, "plan has no subband path");
+  if (plan->input_bits == 32)
+    return fail(DD_ERR_INVALID_ARGUMENT, "the plan takes float input (dd_subband_step_one_f32)");
   const dd_observation &o = plan->obs;
   const dd_sizes &z = plan->sizes;
-  if ((uint64_t)z.nr_samples_subbanding + z.max_shift_step_one > (z.sub_in_pitch << input_log2f(plan)))
+  if ((uint64_t)z.nr_samples_step_one + z.max_shift_step_one > z.nr_dispersed_samples_subbanding)
     return fail(DD_ERR_INVALID_ARGUMENT, "dispersed batch shorter than samples + largest shift");
   dd_status s = geometry_of(plan, DD_STEP_ONE, c, L.g);
   if (s != DD_OK) return s;
@@ -456,7 +489,7 @@ dd_status prepare_step_one(dd_plan *plan, const dd_conf &c, const uint8_t *d_inp
   k.out_dm_pitch = (uint64_t)o.nr_subbands * z.sub_out_pitch;
   k.out_beam_pitch = (uint64_t)o.nr_dms_subbanding * o.nr_subbands * z.sub_out_pitch;
   k.out_group_pitch = z.sub_out_pitch;
-  k.n_samples = z.nr_samples_subbanding;
+  k.n_samples = z.nr_samples_step_one;
   k.n_dms = o.nr_dms_subbanding;
   k.n_groups = o.nr_subbands;
   k.unroll = L.g.unroll;
@@ -467,7 +500,7 @@ dd_status prepare_step_one(dd_plan *plan, const dd_conf &c, const uint8_t *d_inp
   k.first_dm = o.first_dm_subbanding;
   k.dm_step = o.dm_step_subbanding;
   const uint32_t ts = 64 * c.items_d0, td = c.threads_d1 * c.items_d1;
-  L.grid = dim3((z.nr_samples_subbanding + ts - 1) / ts, (o.nr_dms_subbanding + td - 1) / td,
+  L.grid = dim3((z.nr_samples_step_one + ts - 1) / ts, (o.nr_dms_subbanding + td - 1) / td,
                 o.nr_beams * o.nr_subbands);
   L.flush = cps(plan) > 257;
   k.n_tiles_x = L.grid.x;
@@ -485,14 +518,14 @@ dd_status prepare_step_two(dd_plan *plan, const dd_conf &c, const float *d_inter
   if (nr_sbeams == 0) nr_sbeams = o.nr_synthesized_beams - std::min(first_sbeam, o.nr_synthesized_beams);
   if (nr_sbeams == 0 || first_sbeam + nr_sbeams > o.nr_synthesized_beams)
     return fail(DD_ERR_INVALID_ARGUMENT, "synthesized beam range out of bounds");
-  if ((uint64_t)z.nr_samples + z.max_shift_step_two > z.sub_out_pitch)
+  if ((uint64_t)z.nr_samples + z.max_shift_step_two > z.nr_samples_step_one)
     return fail(DD_ERR_INVALID_ARGUMENT, "subbanded batch shorter than samples + largest shift");
   dd_status s = geometry_of(plan, DD_STEP_TWO, c, L.g);
   if (s != DD_OK) return s;
   RowSumF32Params &k = L.k;
   k = RowSumF32Params{};
   k.in = d_intermediate;
-  k.in_floats = z.intermediate_elements;
+  k.in_floats = z.step_two_input_elements;
   k.out = d_output;
   k.ishift = plan->ish_two.dev.ptr;
   k.dm_pitch = plan->ish_two.pitch;
@@ -501,11 +534,12 @@ dd_status prepare_step_two(dd_plan *plan, const dd_conf &c, const float *d_inter
   k.first_sbeam = first_sbeam;
   k.n_rows = o.nr_subbands;
   k.n_coarse = o.nr_dms_subbanding;
-  k.in_row_pitch = z.sub_out_pitch;
-  k.in_coarse_pitch = (uint64_t)o.nr_subbands * z.sub_out_pitch;
-  k.in_beam_pitch = (uint64_t)o.nr_dms_subbanding * o.nr_subbands * z.sub_out_pitch;
+  k.in_row_pitch = z.step_two_in_pitch;
+  k.in_coarse_pitch = (uint64_t)o.nr_subbands * z.step_two_in_pitch;
+  k.in_beam_pitch = (uint64_t)o.nr_dms_subbanding * o.nr_subbands * z.step_two_in_pitch;
   k.out_dm_pitch = z.out_pitch;
   k.out_beam_pitch = (uint64_t)o.nr_dms_subbanding * o.nr_dms * z.out_pitch;
+  k.out_inner_pitch = (uint64_t)o.nr_dms * z.out_pitch;
   k.n_samples = z.nr_samples;
   k.n_dms = o.nr_dms;
   k.unroll = L.g.unroll;
@@ -550,6 +584,103 @@ dd_status prepare_step_two_u16(dd_plan *plan, const dd_conf &c, const void *d_in
   L.k.unroll = unroll;
   L.k.win_floats = win;
   L.k.combine_rows = 65535u / bound >= 4 ? 4 : 65535u / bound >= 2 ? 2 : 1;
+  return DD_OK;
+}
+
+dd_status prepare_single_f32(dd_plan *plan, const dd_conf &c, const float *d_input, float *d_output,
+                             uint32_t first_sbeam, uint32_t nr_sbeams, F32Launch &L) {
+  if (!plan || !d_input || !d_output) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  if (plan->input_bits != 32) return fail(DD_ERR_INVALID_ARGUMENT, "float input needs a plan with input_bits 32");
+  const dd_observation &o = plan->obs;
+  const dd_sizes &z = plan->sizes;
+  if (nr_sbeams == 0) nr_sbeams = o.nr_synthesized_beams - std::min(first_sbeam, o.nr_synthesized_beams);
+  if (nr_sbeams == 0 || first_sbeam + nr_sbeams > o.nr_synthesized_beams)
+    return fail(DD_ERR_INVALID_ARGUMENT, "synthesized beam range out of bounds");
+  if ((uint64_t)z.nr_samples + z.max_shift > z.nr_dispersed_samples)
+    return fail(DD_ERR_INVALID_ARGUMENT, "dispersed batch shorter than samples + largest shift");
+  dd_status s = geometry_of(plan, DD_SINGLE_STEP, c, L.g);
+  if (s != DD_OK) return s;
+  RowSumF32Params &k = L.k;
+  k = RowSumF32Params{};
+  k.in = d_input;
+  k.in_floats = z.input_elements;
+  k.out = d_output;
+  k.ishift = plan->ish_single.dev.ptr;
+  k.dm_pitch = plan->ish_single.pitch;
+  k.map = plan->map_single.ptr;
+  k.map_pitch = padto(o.nr_channels, plan->padding / 4);
+  k.first_sbeam = first_sbeam;
+  k.n_rows = o.nr_channels;
+  k.n_coarse = 1;
+  k.rows = plan->live_single.ptr;
+  k.rows_count = plan->live_single_count.ptr;
+  k.rows_pitch = o.nr_channels;
+  k.in_row_pitch = z.in_pitch;
+  k.in_coarse_pitch = 0;
+  k.in_beam_pitch = (uint64_t)o.nr_channels * z.in_pitch;
+  k.out_dm_pitch = z.out_pitch;
+  k.out_beam_pitch = (uint64_t)o.nr_dms * z.out_pitch;
+  k.out_inner_pitch = 0;
+  k.n_samples = z.nr_samples;
+  k.n_dms = o.nr_dms;
+  k.unroll = L.g.unroll;
+  k.win_floats = L.g.win;
+  k.nontemporal = 1;
+  k.table = plan->shifts.ptr;
+  k.first_dm = o.first_dm;
+  k.dm_step = o.dm_step;
+  const uint32_t ts = 64 * c.items_d0, td = c.threads_d1 * c.items_d1;
+  L.grid = dim3((z.nr_samples + ts - 1) / ts, (o.nr_dms + td - 1) / td, nr_sbeams);
+  k.n_tiles_x = L.grid.x;
+  k.n_tiles_y = L.grid.y;
+  k.n_tiles = L.grid.x * L.grid.y * L.grid.z;
+  return DD_OK;
+}
+
+dd_status prepare_step_one_f32(dd_plan *plan, const dd_conf &c, const float *d_input, float *d_intermediate,
+                               F32Launch &L) {
+  if (!plan || !d_input || !d_intermediate) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  if (!plan->two_step) return fail(DD_ERR_INVALID_ARGUMENT, "plan has no subband path");
+  if (plan->input_bits != 32) return fail(DD_ERR_INVALID_ARGUMENT, "float input needs a plan with input_bits 32");
+  const dd_observation &o = plan->obs;
+  const dd_sizes &z = plan->sizes;
+  if ((uint64_t)z.nr_samples_step_one + z.max_shift_step_one > z.nr_dispersed_samples_subbanding)
+    return fail(DD_ERR_INVALID_ARGUMENT, "dispersed batch shorter than samples + largest shift");
+  dd_status s = geometry_of(plan, DD_STEP_ONE, c, L.g);
+  if (s != DD_OK) return s;
+  RowSumF32Params &k = L.k;
+  k = RowSumF32Params{};
+  k.in = d_input;
+  k.in_floats = z.sub_input_elements;
+  k.out = d_intermediate;
+  k.ishift = plan->ish_one.dev.ptr;
+  k.dm_pitch = plan->ish_one.pitch;
+  k.map = nullptr;  // the input beam is the beam itself
+  k.n_rows = cps(plan);
+  k.n_coarse = o.nr_subbands;
+  k.rows = plan->live_sub.ptr;
+  k.rows_count = plan->live_sub_count.ptr;
+  k.rows_pitch = cps(plan);
+  k.in_row_pitch = z.sub_in_pitch;
+  k.in_coarse_pitch = 0;
+  k.in_beam_pitch = (uint64_t)o.nr_channels * z.sub_in_pitch;
+  k.out_dm_pitch = (uint64_t)o.nr_subbands * z.sub_out_pitch;
+  k.out_beam_pitch = (uint64_t)o.nr_dms_subbanding * o.nr_subbands * z.sub_out_pitch;
+  k.out_inner_pitch = z.sub_out_pitch;
+  k.n_samples = z.nr_samples_step_one;
+  k.n_dms = o.nr_dms_subbanding;
+  k.unroll = L.g.unroll;
+  k.win_floats = L.g.win;
+  k.nontemporal = 0;  // step two re-reads the intermediate
+  k.table = plan->relative.ptr;
+  k.first_dm = o.first_dm_subbanding;
+  k.dm_step = o.dm_step_subbanding;
+  const uint32_t ts = 64 * c.items_d0, td = c.threads_d1 * c.items_d1;
+  L.grid = dim3((z.nr_samples_step_one + ts - 1) / ts, (o.nr_dms_subbanding + td - 1) / td,
+                o.nr_beams * o.nr_subbands);
+  k.n_tiles_x = L.grid.x;
+  k.n_tiles_y = L.grid.y;
+  k.n_tiles = L.grid.x * L.grid.y * L.grid.z;
   return DD_OK;
 }
 
@@ -624,10 +755,6 @@ dd_status dd_plan_create(dd_plan **out, const dd_observation *obs_in, uint32_t p
   p->obs = o;
   p->padding = padding;
   p->two_step = o.nr_dms_subbanding > 0;
-  if (p->two_step && o.downsampling != 1) {
-    delete p;
-    return fail(DD_ERR_UNSUPPORTED, "the subband path needs downsampling 1");
-  }
   (void)hipGetDevice(&p->device);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, p->device) == hipSuccess) p->max_lds = prop.sharedMemPerBlock;
@@ -688,10 +815,16 @@ dd_status dd_plan_create(dd_plan **out, const dd_observation *obs_in, uint32_t p
   if (p->two_step) {
     // The intermediate rows are padded as for 8-bit input whatever inputBits
     // is; include/Dedispersion.hpp:148 pads S1 / (8 / inputBits), which makes
-    // the rows of packed input overlap.
+    // the rows of packed input overlap.  Step one writes S1 / downsampling
+    // samples per row (:119) into rows of pad(S1) (:148); step two reads rows
+    // of pad(S1 / downsampling) (:163, :169).  The two pitches differ when
+    // downsampling > 1, as they do in the reference.
     z.sub_out_pitch = padto(z.nr_samples_subbanding, padding / 4);
     z.intermediate_elements = (uint64_t)o.nr_beams * o.nr_dms_subbanding * o.nr_subbands * z.sub_out_pitch;
     z.sub_output_elements = (uint64_t)o.nr_synthesized_beams * o.nr_dms_subbanding * o.nr_dms * z.out_pitch;
+    z.nr_samples_step_one = z.nr_samples_subbanding / o.downsampling;
+    z.step_two_in_pitch = padto(z.nr_samples_step_one, padding / 4);
+    z.step_two_input_elements = (uint64_t)o.nr_beams * o.nr_dms_subbanding * o.nr_subbands * z.step_two_in_pitch;
   }
 
   // Default tables: no zapped channel, sBeam -> beam sBeam % nrBeams.
@@ -719,11 +852,20 @@ dd_status dd_plan_create(dd_plan **out, const dd_observation *obs_in, uint32_t p
 
 dd_status dd_plan_set_input_bits(dd_plan *plan, uint32_t input_bits) {
   if (!plan) return fail(DD_ERR_INVALID_ARGUMENT, "null plan");
-  if (input_bits != 1 && input_bits != 2 && input_bits != 4 && input_bits != 8)
-    return fail(DD_ERR_INVALID_ARGUMENT, "input_bits must be 1, 2, 4 or 8");
+  if (input_bits != 1 && input_bits != 2 && input_bits != 4 && input_bits != 8 && input_bits != 32)
+    return fail(DD_ERR_INVALID_ARGUMENT, "input_bits must be 1, 2, 4, 8 or 32 (float)");
   plan->input_bits = input_bits;
   size_inputs(plan);
   return DD_OK;
+}
+
+dd_status dd_plan_set_step_one_shifts(dd_plan *plan, dd_step_one_shifts delays) {
+  if (!plan) return fail(DD_ERR_INVALID_ARGUMENT, "null plan");
+  if (delays != DD_STEP_ONE_RELATIVE && delays != DD_STEP_ONE_ABSOLUTE)
+    return fail(DD_ERR_INVALID_ARGUMENT, "step-one delays must be DD_STEP_ONE_RELATIVE or DD_STEP_ONE_ABSOLUTE");
+  if (!plan->two_step) return fail(DD_ERR_INVALID_ARGUMENT, "plan has no subband path");
+  plan->step_one_absolute = delays == DD_STEP_ONE_ABSOLUTE;
+  return rebuild_integer_tables(plan);
 }
 
 dd_status dd_plan_destroy(dd_plan *plan) {
@@ -834,6 +976,21 @@ dd_status dd_subband_step_one(dd_plan *plan, const uint8_t *d_input, float *d_in
   return s != DD_OK ? s : run(L, (hipStream_t)stream);
 }
 
+dd_status dd_dedisperse_f32(dd_plan *plan, const float *d_input, float *d_output, uint32_t first_sbeam,
+                            uint32_t nr_sbeams, void *stream) {
+  if (!plan) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  F32Launch L;
+  dd_status s = prepare_single_f32(plan, plan->conf[DD_SINGLE_STEP], d_input, d_output, first_sbeam, nr_sbeams, L);
+  return s != DD_OK ? s : run(L, (hipStream_t)stream);
+}
+
+dd_status dd_subband_step_one_f32(dd_plan *plan, const float *d_input, float *d_intermediate, void *stream) {
+  if (!plan) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  F32Launch L;
+  dd_status s = prepare_step_one_f32(plan, plan->conf[DD_STEP_ONE], d_input, d_intermediate, L);
+  return s != DD_OK ? s : run(L, (hipStream_t)stream);
+}
+
 dd_status dd_subband_step_two(dd_plan *plan, const float *d_intermediate, float *d_output, uint32_t first_sbeam,
                               uint32_t nr_sbeams, void *stream) {
   if (!plan) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
@@ -852,11 +1009,14 @@ dd_status dd_tune_time(dd_plan *plan, dd_mode mode, const dd_conf *conf, const v
   if (s != DD_OK) return s;
   const dd_conf saved = plan->conf[mode];
   plan->conf[mode] = *conf;
+  const bool float_input = plan->input_bits == 32;
   auto launch = [&]() -> dd_status {
     switch (mode) {
     case DD_SINGLE_STEP:
+      if (float_input) return dd_dedisperse_f32(plan, (const float *)d_input, (float *)d_output, 0, 0, stream);
       return dd_dedisperse(plan, (const uint8_t *)d_input, (float *)d_output, 0, 0, stream);
     case DD_STEP_ONE:
+      if (float_input) return dd_subband_step_one_f32(plan, (const float *)d_input, (float *)d_output, stream);
       return dd_subband_step_one(plan, (const uint8_t *)d_input, (float *)d_output, stream);
     default:
       return dd_subband_step_two(plan, (const float *)d_input, (float *)d_output, 0, 0, stream);

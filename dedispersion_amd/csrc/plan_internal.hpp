@@ -79,7 +79,8 @@ struct dd_plan {
   int device = 0;
   size_t max_lds = 65536;
   bool two_step = false;
-  uint32_t input_bits = 8;  // the reference's inputBits: 1, 2, 4 (packed rows) or 8
+  bool step_one_absolute = false;  // step one applies each channel's own delay (dd_plan_set_step_one_shifts)
+  uint32_t input_bits = 8;  // the reference's inputBits: 1, 2, 4 (packed rows), 8, or 32 (float rows)
   dd_sizes sizes{};
   dd_conf conf[3]{};
 
@@ -142,6 +143,12 @@ dd_status prepare_step_two(dd_plan *plan, const dd_conf &c, const float *d_inter
 // the staged pipeline or a subband's sum could reach 2^16.
 dd_status prepare_step_two_u16(dd_plan *plan, const dd_conf &c, const void *d_intermediate, float *d_output,
                                uint32_t first_sbeam, uint32_t nr_sbeams, F32Launch &L);
+// The single step and step one over float input rows (a plan with
+// input_bits 32): the float row-sum kernels over the live-channel lists.
+dd_status prepare_single_f32(dd_plan *plan, const dd_conf &c, const float *d_input, float *d_output,
+                             uint32_t first_sbeam, uint32_t nr_sbeams, F32Launch &L);
+dd_status prepare_step_one_f32(dd_plan *plan, const dd_conf &c, const float *d_input, float *d_intermediate,
+                               F32Launch &L);
 // Launch a prepared launch (staged or direct kernel).
 dd_status run(const U8Launch &L, hipStream_t stream);
 dd_status run(const F32Launch &L, hipStream_t stream);

@@ -156,6 +156,9 @@ dd_status dd_stream_create(dd_stream **out, dd_plan *plan, uint32_t nr_beams, ui
   *out = nullptr;
   if (!plan->two_step) return fail(DD_ERR_INVALID_ARGUMENT, "streaming runs the subband path: plan has no coarse DMs");
   if (plan->input_bits != 8) return fail(DD_ERR_UNSUPPORTED, "streaming takes 8-bit input only");
+  if (plan->obs.downsampling != 1)
+    return fail(DD_ERR_UNSUPPORTED, "streaming chains step one into step two, whose intermediate pitches differ at "
+                                    "downsampling > 1");
   const dd_observation &o = plan->obs;
   if (o.nr_beams != 1 || o.nr_synthesized_beams != 1)
     return fail(DD_ERR_INVALID_ARGUMENT, "a streaming plan describes one beam (nr_beams = nr_synthesized_beams = 1)");

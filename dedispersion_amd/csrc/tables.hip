@@ -27,11 +27,14 @@ __global__ void build_shifts(const ShiftFormula f, float *out) {
   out[i] = value;  // rows - 1 and the padding stay 0 (:28)
 }
 
-__global__ void relative_shifts(const float *table, uint32_t channels, uint32_t per_subband, float *out) {
+__global__ void relative_shifts(const float *table, uint32_t channels, uint32_t per_subband, uint32_t absolute,
+                                float *out) {
   const uint32_t ch = blockIdx.x * blockDim.x + threadIdx.x;
   if (ch >= channels) return;
   const uint32_t top = (ch / per_subband + 1) * per_subband - 1;
-  out[ch] = __fsub_rn(table[ch], table[top]);  // include/Dedispersion.hpp:124
+  // include/Dedispersion.hpp:124 (the CPU function), or the channel's own
+  // delay, as the OpenCL step-one kernel applies it (:595-597, :731, :733)
+  out[ch] = absolute ? table[ch] : __fsub_rn(table[ch], table[top]);
 }
 
 __global__ void integer_shifts(const float *table, uint32_t rows, uint32_t dms, uint32_t dm_pitch,
@@ -53,11 +56,11 @@ hipError_t launch_build_shifts(const ShiftFormula &f, float *d_table, hipStream_
   return hipGetLastError();
 }
 
-hipError_t launch_relative_shifts(const float *d_table, uint32_t channels, uint32_t per_subband,
+hipError_t launch_relative_shifts(const float *d_table, uint32_t channels, uint32_t per_subband, bool absolute,
                                   float *d_relative, hipStream_t stream) {
   const uint32_t blocks = (channels + 255) / 256;
   hipLaunchKernelGGL(relative_shifts, dim3(blocks), dim3(256), 0, stream, d_table, channels, per_subband,
-                     d_relative);
+                     absolute ? 1u : 0u, d_relative);
   return hipGetLastError();
 }
 

@@ -137,12 +137,13 @@ def batched_beams(nr_beams: int = 40, seed: int = 4) -> List[Workload]:
 
 
 def tiny(channels=16, samples=64, dms=8, step=2.0, subbands=1, coarse=0, coarse_step=0.0, beams=1, sbeams=1,
-         min_freq=1400.0, bandwidth=2.0, zapped=(), seed=5) -> Workload:
-    """Small shapes for exhaustive parity checks."""
+         min_freq=1400.0, bandwidth=2.0, zapped=(), seed=5, first=0.0, coarse_first=0.0) -> Workload:
+    """Small shapes for exhaustive parity checks (first, coarse_first: the
+    first DM of the fine / single-step and of the coarse DM range)."""
     obs = Observation(
         nr_channels=channels, min_freq=min_freq, channel_bandwidth=bandwidth, sampling_time=SAMPLING_TIME,
-        nr_samples_per_batch=samples, nr_dms=dms, first_dm=0.0, dm_step=step, nr_subbands=subbands,
-        nr_dms_subbanding=coarse, first_dm_subbanding=0.0, dm_step_subbanding=coarse_step,
+        nr_samples_per_batch=samples, nr_dms=dms, first_dm=first, dm_step=step, nr_subbands=subbands,
+        nr_dms_subbanding=coarse, first_dm_subbanding=coarse_first, dm_step_subbanding=coarse_step,
         nr_beams=beams, nr_synthesized_beams=sbeams,
     )
     return Workload(f"tiny-{channels}ch-{dms}dm-{samples}s", obs, zapped=list(zapped), noise="uniform", seed=seed)

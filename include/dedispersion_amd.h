@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define DD_ABI_VERSION 3
+#define DD_ABI_VERSION 4
 
 typedef enum dd_status {
   DD_OK = 0,
@@ -104,7 +104,8 @@ typedef enum dd_mode {
 } dd_mode;
 
 /* Sizes of the device buffers of a plan, in elements of their type, using
- * the reference layouts:
+ * the reference layouts (input elements are bytes, or floats for float
+ * input):
  *   input        uint8 [beams][C][in_pitch]              (include/Dedispersion.hpp:90)
  *   output       float [sBeams][D][out_pitch]            (include/Dedispersion.hpp:105)
  *   step-one in  uint8 [beams][C][sub_in_pitch]          (include/Dedispersion.hpp:133)
@@ -115,9 +116,9 @@ typedef struct dd_sizes {
   uint32_t nr_dispersed_samples;       /* S_disp */
   uint32_t nr_samples_subbanding;      /* S1 */
   uint32_t nr_dispersed_samples_subbanding; /* S1_disp */
-  uint64_t in_pitch;                   /* bytes per channel row, single step */
+  uint64_t in_pitch;                   /* input elements per channel row, single step */
   uint64_t out_pitch;                  /* floats per DM row */
-  uint64_t sub_in_pitch;               /* bytes per channel row, step one */
+  uint64_t sub_in_pitch;               /* input elements per channel row, step one */
   uint64_t sub_out_pitch;              /* floats per subband row of the intermediate */
   uint64_t input_elements;             /* single-step input */
   uint64_t output_elements;            /* single-step output */
@@ -128,6 +129,16 @@ typedef struct dd_sizes {
   uint32_t max_shift_step_one;
   uint32_t max_shift_step_two;
   uint32_t input_bits;                 /* bits per input sample (dd_plan_set_input_bits) */
+  /* Downsampling in the subband path (include/Dedispersion.hpp:119, 148,
+   * 163, 169): step one writes S1 / downsampling samples per row into rows of
+   * sub_out_pitch = pad(S1) floats, while step two reads rows of
+   * step_two_in_pitch = pad(S1 / downsampling) floats.  The two agree only
+   * at downsampling 1; each step follows its own CPU function, so a caller
+   * chaining them at downsampling > 1 re-lays the rows in between, as a
+   * caller of the reference must. */
+  uint32_t nr_samples_step_one;        /* S1 / downsampling */
+  uint64_t step_two_in_pitch;          /* floats per subband row of step two's input */
+  uint64_t step_two_input_elements;    /* step-two input */
 } dd_sizes;
 
 typedef struct dd_plan dd_plan;
@@ -148,8 +159,16 @@ dd_status dd_plan_destroy(dd_plan *plan);
 dd_status dd_plan_get_sizes(const dd_plan *plan, dd_sizes *sizes);
 
 /* Bits per input sample, the reference's `inputBits` argument
- * (include/Dedispersion.hpp:59-60, :63-64): 8 (the default) or 1, 2, 4 for
- * packed rows.  Packed rows hold 8 / input_bits samples per byte, sample x
+ * (include/Dedispersion.hpp:59-60, :63-64): 8 (the default), 1, 2, 4 for
+ * packed rows, or 32 for float input (the reference's templates with
+ * I = float).  Float rows are pad(S_disp, padding / 4) floats; in_pitch,
+ * sub_in_pitch and the input sizes then count floats, and only
+ * dd_dedisperse_f32 and dd_subband_step_one_f32 take the plan's input (the
+ * 8-bit entry points refuse it, and the float ones refuse other plans).  A
+ * float row sum adds the live channels in order into a float accumulator
+ * that starts at 0, the reference's sequential float sum (:80-106,
+ * :123-146), so results are bit-identical to it for any float input.
+ * Batches and streams refuse float plans.  Packed rows (1, 2, 4):  Packed rows hold 8 / input_bits samples per byte, sample x
  * in the field at bit (x % (8 / input_bits)) * input_bits of byte
  * x / (8 / input_bits), least significant bit first, read as an unsigned
  * value (include/Dedispersion.hpp:94-101 and :137-144).  A row of the
@@ -165,6 +184,26 @@ dd_status dd_plan_get_sizes(const dd_plan *plan, dd_sizes *sizes);
  * configurations with local 0, 2 or 3 run as local 1 with unroll capped at
  * 2 x threads_d1.  Streams (dd_stream_create) take 8-bit plans only. */
 dd_status dd_plan_set_input_bits(dd_plan *plan, uint32_t input_bits);
+
+/* Step one's delay per channel (a plan with a subband path):
+ *   DD_STEP_ONE_RELATIVE (the default): the channel's delay minus that of
+ *     its subband's top channel, (unsigned)((first + dm * step) *
+ *     (shifts[ch] - shifts[top])), the reference's CPU function
+ *     (include/Dedispersion.hpp:124); the coarse-DM delay between subbands
+ *     is then applied nowhere, so a two-step output dedisperses correctly
+ *     only at coarse DM 0;
+ *   DD_STEP_ONE_ABSOLUTE: the channel's own delay, (unsigned)((first + dm *
+ *     step) * shifts[ch]), the reference's OpenCL step-one kernel
+ *     (include/Dedispersion.hpp:595-597, :731, :733), which makes step one
+ *     plus step two a dedispersion at DM coarse + fine.
+ * The same kernels run either way; only the delay table differs.  (The
+ * reference's OpenCL kernel bakes firstDM and DMStep into its source with
+ * six decimals, :208-219; this engine uses the float values.) */
+typedef enum dd_step_one_shifts {
+  DD_STEP_ONE_RELATIVE = 0,
+  DD_STEP_ONE_ABSOLUTE = 1
+} dd_step_one_shifts;
+dd_status dd_plan_set_step_one_shifts(dd_plan *plan, dd_step_one_shifts delays);
 
 /* Kernel configuration per mode (defaults are chosen at creation). */
 dd_status dd_plan_set_conf(dd_plan *plan, dd_mode mode, const dd_conf *conf);
@@ -197,10 +236,21 @@ dd_status dd_dedisperse(dd_plan *plan, const uint8_t *d_input, float *d_output,
                         uint32_t first_synthesized_beam, uint32_t nr_synthesized_beams,
                         void *stream);
 /* Step one: channels into subbands at the coarse DMs, with the shift
- * relative to each subband's top channel (include/Dedispersion.hpp:124).
+ * relative to each subband's top channel (include/Dedispersion.hpp:124), or
+ * the channel's own (dd_plan_set_step_one_shifts).
  * Replaces `dedispersionStepOne` (include/Dedispersion.hpp:543-882). */
 dd_status dd_subband_step_one(dd_plan *plan, const uint8_t *d_input, float *d_intermediate,
                               void *stream);
+/* The single step and step one over float input rows (a plan with
+ * dd_plan_set_input_bits(plan, 32)); same layouts and semantics as
+ * dd_dedisperse and dd_subband_step_one, with I = float
+ * (include/Dedispersion.hpp:59-60, generators :63-64 with inputDataType
+ * "float"). */
+dd_status dd_dedisperse_f32(dd_plan *plan, const float *d_input, float *d_output,
+                            uint32_t first_synthesized_beam, uint32_t nr_synthesized_beams,
+                            void *stream);
+dd_status dd_subband_step_one_f32(dd_plan *plan, const float *d_input, float *d_intermediate,
+                                  void *stream);
 /* Step two: subbands at the fine DMs.  Replaces `dedispersionStepTwo`
  * (include/Dedispersion.hpp:884-1122). */
 dd_status dd_subband_step_two(dd_plan *plan, const float *d_intermediate, float *d_output,

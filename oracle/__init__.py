@@ -37,6 +37,7 @@ class OrcObs(ctypes.Structure):
         ("first_dm_sub", c_float),
         ("dm_step_sub", c_float),
         ("input_bits", c_uint32),
+        ("step_one_absolute", c_uint32),
     ]
 
 
@@ -103,11 +104,14 @@ def pad(v: int, k: int) -> int:
 class Oracle:
     """The reference's CPU functions over one observation (numpy in/out)."""
 
-    def __init__(self, obs, padding: int, threads: int = 0, input_bits: int = 8):
+    def __init__(self, obs, padding: int, threads: int = 0, input_bits: int = 8, step_one_absolute: bool = False):
         """obs: a dedispersion_amd.Observation-like object with the same field
         names; dispersed sizes of 0 are derived with the reference's rules.
-        input_bits: the reference's inputBits (1, 2, 4 = packed rows, 8)."""
-        assert input_bits in (1, 2, 4, 8)
+        input_bits: the reference's inputBits (1, 2, 4 = packed rows, 8, or 32
+        = float input).  step_one_absolute: step one applies each channel's
+        own delay (the reference's OpenCL kernel) instead of its delay
+        relative to the subband's top channel (the CPU function)."""
+        assert input_bits in (1, 2, 4, 8, 32)
         self.lib = load()
         self.padding = padding
         self.input_bits = input_bits
@@ -129,6 +133,7 @@ class Oracle:
         o.first_dm_sub = obs.first_dm_subbanding
         o.dm_step_sub = obs.dm_step_subbanding
         o.input_bits = input_bits
+        o.step_one_absolute = 1 if step_one_absolute else 0
         self.o = o
         self.shifts = self.get_shifts()
         self.shifts2 = self.get_shifts_step_two()
@@ -145,8 +150,12 @@ class Oracle:
 
     # sizes (elements)
     @property
+    def input_dtype(self):
+        return np.float32 if self.input_bits == 32 else np.uint8
+
+    @property
     def in_pitch(self):
-        """Bytes per channel row of the single-step input."""
+        """Input elements (bytes, or floats) per channel row of the single-step input."""
         return self.lib.orc_input_pitch(self.o.nr_samples_per_dispersed_batch, self.input_bits, self.padding)
 
     @property
@@ -155,12 +164,19 @@ class Oracle:
 
     @property
     def sub_in_pitch(self):
-        """Bytes per channel row of the step-one input."""
+        """Input elements (bytes, or floats) per channel row of the step-one input."""
         return self.lib.orc_input_pitch(self.o.nr_samples_per_dispersed_batch_sub, self.input_bits, self.padding)
 
     @property
     def sub_out_pitch(self):
+        """Floats per row of step one's output: pad(S1) (include/Dedispersion.hpp:148)."""
         return pad(self.o.nr_samples_per_batch_sub, self.padding // 4)
+
+    @property
+    def sub_two_in_pitch(self):
+        """Floats per row of step two's input: pad(S1 / downsampling) (include/Dedispersion.hpp:169),
+        which equals sub_out_pitch only when downsampling is 1."""
+        return pad(self.o.nr_samples_per_batch_sub // self.o.downsampling, self.padding // 4)
 
     def get_shifts(self) -> np.ndarray:
         out = np.zeros(pad(self.o.nr_channels, self.padding // 4), dtype=np.float32)
@@ -187,7 +203,8 @@ class Oracle:
         z = np.zeros(pad(C, self.padding // 4), dtype=np.uint32) if zapped is None else np.ascontiguousarray(zapped, dtype=np.uint32)
         m = self.default_map(C) if beam_mapping is None else np.ascontiguousarray(beam_mapping, dtype=np.uint32)
         s = self.shifts if shifts is None else np.ascontiguousarray(shifts, dtype=np.float32)
-        inp = np.ascontiguousarray(inp, dtype=np.uint8)
+        assert np.asarray(inp).dtype == self.input_dtype, "input type of the observation's input_bits"
+        inp = np.ascontiguousarray(inp)
         assert inp.size >= o.nr_beams * C * self.in_pitch
         out = np.zeros(o.nr_synthesized_beams * o.nr_dms * self.out_pitch, dtype=np.float32)
         r0, r1 = (0, o.nr_synthesized_beams * o.nr_dms) if rows is None else rows
@@ -200,7 +217,8 @@ class Oracle:
         C = o.nr_channels
         z = np.zeros(pad(C, self.padding // 4), dtype=np.uint32) if zapped is None else np.ascontiguousarray(zapped, dtype=np.uint32)
         s = self.shifts if shifts is None else np.ascontiguousarray(shifts, dtype=np.float32)
-        inp = np.ascontiguousarray(inp, dtype=np.uint8)
+        assert np.asarray(inp).dtype == self.input_dtype, "input type of the observation's input_bits"
+        inp = np.ascontiguousarray(inp)
         assert inp.size >= o.nr_beams * C * self.sub_in_pitch
         out = np.zeros(o.nr_beams * o.nr_dms_sub * o.nr_subbands * self.sub_out_pitch, dtype=np.float32)
         r0, r1 = (0, o.nr_beams * o.nr_dms_sub) if rows is None else rows
@@ -213,7 +231,7 @@ class Oracle:
         m = self.default_map(o.nr_subbands) if beam_mapping is None else np.ascontiguousarray(beam_mapping, dtype=np.uint32)
         s = self.shifts2 if shifts is None else np.ascontiguousarray(shifts, dtype=np.float32)
         inp = np.ascontiguousarray(inp, dtype=np.float32)
-        assert inp.size >= o.nr_beams * o.nr_dms_sub * o.nr_subbands * self.sub_out_pitch
+        assert inp.size >= o.nr_beams * o.nr_dms_sub * o.nr_subbands * self.sub_two_in_pitch
         out = np.zeros(o.nr_synthesized_beams * o.nr_dms_sub * o.nr_dms * self.out_pitch, dtype=np.float32)
         r0, r1 = (0, o.nr_synthesized_beams * o.nr_dms_sub * o.nr_dms) if rows is None else rows
         self.lib.orc_step_two_mt(ctypes.byref(o), _u32(m), inp.ctypes.data, _f32(out), _f32(s), self.padding,

@@ -105,16 +105,6 @@ uint32_t orc_shift(float first_dm, float dm_step, uint32_t dm, float table) {
   return (uint32_t)delay;
 }
 
-static uint32_t orc_bits(const orc_obs *o) {
-  return (o->input_bits == 0 || o->input_bits >= 8) ? 8u : o->input_bits;
-}
-
-uint64_t orc_input_pitch(uint32_t dispersed_samples, uint32_t input_bits, uint32_t padding) {
-  if (input_bits == 0 || input_bits >= 8) return orc_pad(dispersed_samples, padding);
-  const uint32_t per_byte = 8 / input_bits;
-  return orc_pad((dispersed_samples + per_byte - 1) / per_byte, padding);
-}
-
 /* include/Dedispersion.hpp:94-101: the inputBits-bit field of sample x of a
  * packed row, copied bit by bit, least significant bit first, into a value
  * whose higher bits are zero.  Byte x / (8 / inputBits) holds the sample,
@@ -128,20 +118,46 @@ static uint32_t orc_packed_sample(const uint8_t *row, uint64_t x, uint32_t bits)
   return value;
 }
 
+/* Bits per input sample: 1, 2, 4 (packed bytes), 8 (bytes) or 32 (floats,
+ * the reference's templates with I = float). */
+static uint32_t orc_bits(const orc_obs *o) {
+  return o->input_bits == 0 ? 8u : o->input_bits;
+}
+
+/* Elements of the input type per row: pad(S_disp, padding / sizeof(I)) for
+ * whole elements (include/Dedispersion.hpp:90, :133), packed bytes rounded up
+ * for sub-byte samples (:97, :140 round down).  padding in bytes. */
+uint64_t orc_input_pitch(uint32_t dispersed_samples, uint32_t input_bits, uint32_t padding) {
+  if (input_bits == 32) return orc_pad(dispersed_samples, padding / sizeof(float));
+  if (input_bits == 0 || input_bits >= 8) return orc_pad(dispersed_samples, padding);
+  const uint32_t per_byte = 8 / input_bits;
+  return orc_pad((dispersed_samples + per_byte - 1) / per_byte, padding);
+}
+
+/* Sample x of an input row as the accumulator type (:90-91 for whole
+ * elements of uint8 or float input, :94-101 for packed fields). */
+static float orc_sample(const void *input, uint64_t row_offset, uint64_t x, uint32_t bits) {
+  if (bits == 32) return ((const float *)input)[row_offset + x];
+  const uint8_t *row = (const uint8_t *)input + row_offset;
+  if (bits == 8) return (float)row[x];
+  return (float)orc_packed_sample(row, x, bits);
+}
+
+
 /* include/Dedispersion.hpp:70-109: the inputBits >= 8 branch (:88-91) and the
  * sub-byte branch (:92-103).  The sub-byte branch of the reference indexes
  * the input with the mapped beam number alone (:97, no beam stride), which
  * only reads the right row for beam 0; here both branches use the beam
  * stride of :90. */
 void orc_dedispersion(const orc_obs *o, const uint32_t *zapped,
-                      const uint32_t *beam_mapping, const uint8_t *input,
+                      const uint32_t *beam_mapping, const void *input,
                       float *output, const float *shifts, uint32_t padding,
                       uint64_t row_begin, uint64_t row_end) {
   const uint32_t C = o->nr_channels;
   const uint32_t D = o->nr_dms;
   const uint32_t samples = o->nr_samples_per_batch / o->downsampling;      /* :76 */
   const uint32_t bits = orc_bits(o);
-  const uint64_t in_pitch = orc_input_pitch(o->nr_samples_per_dispersed_batch, bits, padding / sizeof(uint8_t));
+  const uint64_t in_pitch = orc_input_pitch(o->nr_samples_per_dispersed_batch, bits, padding);
   const uint64_t out_pitch = orc_pad(samples, padding / sizeof(float));
   const uint64_t map_pitch = orc_pad(C, padding / sizeof(uint32_t));
   for (uint64_t row = row_begin; row < row_end; row++) {
@@ -153,11 +169,8 @@ void orc_dedispersion(const orc_obs *o, const uint32_t *zapped,
         uint32_t shift = orc_shift(o->first_dm, o->dm_step, dm, shifts[channel]); /* :81 */
         if (zapped[channel] != 0) continue;                              /* :83 */
         uint64_t beam = beam_mapping[sbeam * map_pitch + channel];
-        const uint8_t *row = input + beam * C * in_pitch + (uint64_t)channel * in_pitch; /* :90 */
-        if (bits == 8)
-          acc += (float)row[sample + shift];
-        else
-          acc += (float)orc_packed_sample(row, (uint64_t)sample + shift, bits); /* :94-101 */
+        const uint64_t row = beam * C * in_pitch + (uint64_t)channel * in_pitch; /* :90 */
+        acc += orc_sample(input, row, (uint64_t)sample + shift, bits);   /* :90, :94-101 */
       }
       output[sbeam * D * out_pitch + (uint64_t)dm * out_pitch + sample] = acc; /* :105 */
     }
@@ -167,9 +180,12 @@ void orc_dedispersion(const orc_obs *o, const uint32_t *zapped,
 /* include/Dedispersion.hpp:111-153: the inputBits >= 8 branch (:131-134)
  * and the sub-byte branch (:135-146).  The output rows are padded as the
  * 8-bit case pads them; the reference pads S1 / (8 / inputBits) there (:148),
- * which makes consecutive sub-byte rows overlap. */
+ * which makes consecutive sub-byte rows overlap.  A channel's delay is its
+ * table entry minus that of its subband's top channel (:124); with
+ * step_one_absolute it is the table entry itself, the delay the reference's
+ * OpenCL step-one kernel applies (:595-597, :731, :733). */
 void orc_step_one(const orc_obs *o, const uint32_t *zapped,
-                  const uint8_t *input, float *output, const float *shifts,
+                  const void *input, float *output, const float *shifts,
                   uint32_t padding, uint64_t row_begin, uint64_t row_end) {
   const uint32_t C = o->nr_channels;
   const uint32_t Nsb = o->nr_subbands;
@@ -177,7 +193,7 @@ void orc_step_one(const orc_obs *o, const uint32_t *zapped,
   const uint32_t Dc = o->nr_dms_sub;
   const uint32_t samples = o->nr_samples_per_batch_sub / o->downsampling; /* :119 */
   const uint32_t bits = orc_bits(o);
-  const uint64_t in_pitch = orc_input_pitch(o->nr_samples_per_dispersed_batch_sub, bits, padding / sizeof(uint8_t));
+  const uint64_t in_pitch = orc_input_pitch(o->nr_samples_per_dispersed_batch_sub, bits, padding);
   const uint64_t out_pitch = orc_pad(o->nr_samples_per_batch_sub, padding / sizeof(float));
   for (uint64_t row = row_begin; row < row_end; row++) {
     const uint64_t beam = row / Dc;                                      /* :113 */
@@ -187,14 +203,12 @@ void orc_step_one(const orc_obs *o, const uint32_t *zapped,
       for (uint32_t sample = 0; sample < samples; sample++) {
         float acc = 0.0f;
         for (uint32_t channel = subband * cps; channel < (subband + 1) * cps; channel++) { /* :122 */
-          float relative = shifts[channel] - top;                        /* :124 */
-          uint32_t shift = orc_shift(o->first_dm_sub, o->dm_step_sub, dm, relative);
+          float delay = o->step_one_absolute ? shifts[channel]             /* :595-597 */
+                                             : shifts[channel] - top;    /* :124 */
+          uint32_t shift = orc_shift(o->first_dm_sub, o->dm_step_sub, dm, delay);
           if (zapped[channel] != 0) continue;                            /* :126 */
-          const uint8_t *row = input + beam * C * in_pitch + (uint64_t)channel * in_pitch; /* :133, :140 */
-          if (bits == 8)
-            acc += (float)row[sample + shift];
-          else
-            acc += (float)orc_packed_sample(row, (uint64_t)sample + shift, bits); /* :137-144 */
+          const uint64_t row = beam * C * in_pitch + (uint64_t)channel * in_pitch; /* :133, :140 */
+          acc += orc_sample(input, row, (uint64_t)sample + shift, bits); /* :133, :137-144 */
         }
         output[beam * Dc * Nsb * out_pitch + (uint64_t)dm * Nsb * out_pitch +
                (uint64_t)subband * out_pitch + sample] = acc;           /* :148 */
@@ -250,10 +264,10 @@ typedef struct mt_job {
 static void *mt_run(void *arg) {
   mt_job *j = (mt_job *)arg;
   if (j->kind == 0)
-    orc_dedispersion(j->o, j->zapped, j->beam_mapping, (const uint8_t *)j->input,
+    orc_dedispersion(j->o, j->zapped, j->beam_mapping, j->input,
                      j->output, j->shifts, j->padding, j->begin, j->end);
   else if (j->kind == 1)
-    orc_step_one(j->o, j->zapped, (const uint8_t *)j->input, j->output,
+    orc_step_one(j->o, j->zapped, j->input, j->output,
                  j->shifts, j->padding, j->begin, j->end);
   else
     orc_step_two(j->o, j->beam_mapping, (const float *)j->input, j->output,
@@ -279,7 +293,7 @@ static void mt_dispatch(mt_job proto, int nthreads) {
 }
 
 void orc_dedispersion_mt(const orc_obs *o, const uint32_t *zapped,
-                         const uint32_t *beam_mapping, const uint8_t *input,
+                         const uint32_t *beam_mapping, const void *input,
                          float *output, const float *shifts, uint32_t padding,
                          uint64_t row_begin, uint64_t row_end, int nthreads) {
   mt_job j = {0, o, zapped, beam_mapping, input, output, shifts, padding, row_begin, row_end};
@@ -287,7 +301,7 @@ void orc_dedispersion_mt(const orc_obs *o, const uint32_t *zapped,
 }
 
 void orc_step_one_mt(const orc_obs *o, const uint32_t *zapped,
-                     const uint8_t *input, float *output, const float *shifts,
+                     const void *input, float *output, const float *shifts,
                      uint32_t padding, uint64_t row_begin, uint64_t row_end,
                      int nthreads) {
   mt_job j = {1, o, zapped, NULL, input, output, shifts, padding, row_begin, row_end};

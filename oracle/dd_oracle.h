@@ -63,8 +63,12 @@ typedef struct orc_obs {
   float first_dm_sub;
   float dm_step_sub;
   /* The reference's inputBits argument (include/Dedispersion.hpp:59-60):
-   * 1, 2 or 4 = packed samples, 8 (or 0) = one byte per sample. */
+   * 1, 2 or 4 = packed samples, 8 (or 0) = one byte per sample, 32 = float
+   * samples (the templates with I = float). */
   uint32_t input_bits;
+  /* Step one's channel delay: 0 = relative to the subband's top channel
+   * (the CPU function, :124), 1 = absolute (the OpenCL kernel, :595-597). */
+  uint32_t step_one_absolute;
 } orc_obs;
 
 /* isa::utils::pad: round v up to a multiple of k; k == 0 means no padding. */
@@ -89,23 +93,24 @@ uint32_t orc_dispersed_samples(uint32_t samples, float shift0, float first_dm,
 /* The per-(channel, DM) integer shift of include/Dedispersion.hpp:81,124,168. */
 uint32_t orc_shift(float first_dm, float dm_step, uint32_t dm, float table);
 
-/* Bytes per channel row of the input: pad(S_disp / (8 / inputBits), padding)
- * as include/Dedispersion.hpp:90,97,133,140 pad it, with the division
- * rounded up so that a row holds all S_disp samples. */
+/* Input elements per channel row: pad(S_disp / (8 / inputBits), padding)
+ * bytes as include/Dedispersion.hpp:90,97,133,140 pad them, with the
+ * division rounded up so that a row holds all S_disp samples; for float
+ * input (inputBits 32) pad(S_disp, padding / 4) floats.  padding in bytes. */
 uint64_t orc_input_pitch(uint32_t dispersed_samples, uint32_t input_bits, uint32_t padding);
 
 /* include/Dedispersion.hpp:70-109.
  * Rows [row_begin, row_end) of the flattened (sBeam, dm) output are
  * computed; pass 0 and sBeams*D for the whole output. */
 void orc_dedispersion(const orc_obs *o, const uint32_t *zapped,
-                      const uint32_t *beam_mapping, const uint8_t *input,
+                      const uint32_t *beam_mapping, const void *input,
                       float *output, const float *shifts, uint32_t padding,
                       uint64_t row_begin, uint64_t row_end);
 
-/* include/Dedispersion.hpp:111-153.  Rows are the flattened
- * (beam, coarse dm) pairs. */
+/* include/Dedispersion.hpp:111-153 (the delay per o->step_one_absolute).
+ * Rows are the flattened (beam, coarse dm) pairs. */
 void orc_step_one(const orc_obs *o, const uint32_t *zapped,
-                  const uint8_t *input, float *output, const float *shifts,
+                  const void *input, float *output, const float *shifts,
                   uint32_t padding, uint64_t row_begin, uint64_t row_end);
 
 /* include/Dedispersion.hpp:155-176.  Rows are the flattened
@@ -117,11 +122,11 @@ void orc_step_two(const orc_obs *o, const uint32_t *beam_mapping,
 /* Same results as the three functions above, rows split over nthreads
  * POSIX threads (used only to time the CPU baseline). */
 void orc_dedispersion_mt(const orc_obs *o, const uint32_t *zapped,
-                         const uint32_t *beam_mapping, const uint8_t *input,
+                         const uint32_t *beam_mapping, const void *input,
                          float *output, const float *shifts, uint32_t padding,
                          uint64_t row_begin, uint64_t row_end, int nthreads);
 void orc_step_one_mt(const orc_obs *o, const uint32_t *zapped,
-                     const uint8_t *input, float *output, const float *shifts,
+                     const void *input, float *output, const float *shifts,
                      uint32_t padding, uint64_t row_begin, uint64_t row_end,
                      int nthreads);
 void orc_step_two_mt(const orc_obs *o, const uint32_t *beam_mapping,

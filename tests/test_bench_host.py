@@ -45,14 +45,34 @@ def test_measured_traffic_matches_only_the_profiled_conf():
     assert bench.measured_traffic("no-such-workload", "step_two_batch", "0 1 4 64 4 1 16 8 1") is None
 
 
-@pytest.mark.parametrize("mode,w", [
-    ("single", workloads.tiny(channels=32, samples=256, dms=16, step=1.0, zapped=(3,))),
-    ("two-step", workloads.tiny(channels=32, samples=256, dms=4, step=0.5, subbands=4, coarse=4, coarse_step=2.0)),
+@pytest.mark.parametrize("mode,ws", [
+    ("single", [workloads.tiny(channels=32, samples=256, dms=16, step=1.0, zapped=(3,))]),
+    ("two-step", [workloads.tiny(channels=32, samples=256, dms=4, step=0.5, subbands=4, coarse=4, coarse_step=2.0),
+                  workloads.tiny(channels=32, samples=512, dms=4, step=0.5, subbands=4, coarse=8, coarse_step=2.0,
+                                 seed=6)]),
 ])
-def test_cpu_baseline_record(mode, w):
-    rec = bench.cpu_baseline(w, mode, 0.2)
+def test_cpu_baseline_record(mode, ws):
+    rec = bench.cpu_baseline(ws, mode, 0.2)
     assert rec["kind"] == "port" and rec["unit"] == "DM-trial*samples/s"
-    assert 1 <= rec["cores"] <= 16 and rec["value"] > 0.0
-    assert w.name in rec["sample"] and "pass(es)" in rec["sample"]
+    host = rec["host"]
+    assert rec["cores"] == host["threads"] and 1 <= host["threads"] <= host["affinity_cpus"]
+    assert host["model"] and host["logical_cpus"] >= host["affinity_cpus"]
+    assert rec["value"] > 0.0
+    if mode == "two-step":
+        assert "of each of the 2 beams of the mix" in rec["sample"]
+    else:
+        assert ws[0].name in rec["sample"] and "pass(es)" in rec["sample"]
     one = rec["single_thread"]
-    assert one["cores"] == 1 and one["value"] > 0.0 and w.name in one["sample"]
+    assert one["cores"] == 1 and one["value"] > 0.0
+
+
+def test_host_cpu_threads_follow_the_process_share(monkeypatch):
+    """The baseline uses every CPU the process may run on, or the per-process
+    share the environment sets when that is smaller."""
+    affinity = len(os.sched_getaffinity(0))
+    monkeypatch.delenv("OMP_NUM_THREADS", raising=False)
+    assert bench.host_cpu()["threads"] == affinity
+    monkeypatch.setenv("OMP_NUM_THREADS", "1")
+    assert bench.host_cpu()["threads"] == 1
+    monkeypatch.setenv("OMP_NUM_THREADS", str(affinity + 100))
+    assert bench.host_cpu()["threads"] == affinity

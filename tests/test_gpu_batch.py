@@ -228,8 +228,9 @@ def test_batch_rejects_unstaged_configuration(torch_cuda):
 def test_batched_beams_40_match_per_beam_launches(torch_cuda, chunks):
     """The 40-beam uneven workload: the graph-replayed batch (persistent, or
     pipelined over two streams) equals one launch per beam and step, bit for
-    bit, on every beam; a few rows of two beams are checked against the
-    oracle directly."""
+    bit, on every beam; one whole coarse-DM band of every beam (its step-one
+    rows over all subbands and its step-two rows over all fine DMs) is
+    checked against the oracle directly."""
     torch = torch_cuda
     ws = workloads.batched_beams()
     bs = BeamSet(torch, ws)
@@ -261,23 +262,26 @@ def test_batched_beams_40_match_per_beam_launches(torch_cuda, chunks):
             wm = ref_mid[j].view(-1, z.sub_out_pitch)[:, : z.nr_samples_subbanding]
             assert torch.equal(gm.to(torch.float32), wm), f"beam {j}: intermediate differs"
         del ref_mid, ref_out
-        for j in (0, 7):
-            plan = bs.plans[j]
+        # one full coarse-DM band of every beam against the oracle: step one
+        # (all subbands) and step two (all fine DMs) of coarse DM c
+        for j, plan in enumerate(bs.plans):
             w = ws[j]
             o, z = w.obs, plan.sizes
+            c = (7 * j + 3) % o.nr_dms_subbanding
             orc = oracle.Oracle(o, bs.padding, threads=host_threads())
             flags = zapped_flags(plan, w.zapped)
             inp = bs.inputs[j].cpu().numpy()
-            raw = bs.mids[j].cpu().numpy()
-            mid = raw.view(np.uint16)[: raw.size].astype(np.float32)
-            c = o.nr_dms_subbanding - 2
-            want = orc.step_one(inp, flags, rows=(c, c + 2))
-            got = mid.reshape(1, o.nr_dms_subbanding, o.nr_subbands, z.sub_out_pitch)
-            compare_rows(got[:, c:c + 2], want[:, c:c + 2], z.nr_samples_subbanding, f"beam {j} step one")
+            n = bs.mids[j].numel()
+            band = bs.mids[j].view(torch.int16)[:n].view(o.nr_dms_subbanding, o.nr_subbands, z.sub_out_pitch)[c]
+            band = (band.to(torch.int32) & 0xFFFF).to(torch.float32).cpu().numpy()
+            want = orc.step_one(inp, flags, rows=(c, c + 1))
+            compare_rows(band[None, None], want[:, c:c + 1], z.nr_samples_subbanding, f"beam {j} step one, coarse DM {c}")
+            mid = np.zeros(n, dtype=np.float32)  # only band c is read by the rows below
+            mid.reshape(o.nr_dms_subbanding, o.nr_subbands, z.sub_out_pitch)[c] = band
             r0, r1 = c * o.nr_dms, (c + 1) * o.nr_dms
             want = orc.step_two(mid, rows=(r0, r1))
-            got = bs.outs[j].cpu().numpy().reshape(1, o.nr_dms_subbanding * o.nr_dms, z.out_pitch)
-            compare_rows(got[:, r0:r1], want[:, r0:r1], o.nr_samples_per_batch, f"beam {j} step two")
+            got = bs.outs[j].view(o.nr_dms_subbanding * o.nr_dms, z.out_pitch)[r0:r1].cpu().numpy()
+            compare_rows(got[None], want[:, r0:r1], o.nr_samples_per_batch, f"beam {j} step two, coarse DM {c}")
     finally:
         bs.close()
         torch.cuda.empty_cache()

@@ -1,9 +1,9 @@
 """Seeded random shapes on the device against the oracle, bit for bit: the
 single step over 8-bit and packed rows, the two-step path and batches of
 uneven two-step jobs, each over shapes drawn from a splitmix64 stream
-(channels, samples, DMs, DM steps, band, zapped channels, beams and their
-mapping, padding) and a configuration drawn from the valid ones.  The
-fixed-shape tests sweep configurations; these sweep shapes."""
+(channels, samples, DMs, first DMs and DM steps, band, zapped channels,
+beams and their mapping, padding) and a configuration drawn from the valid
+ones.  The fixed-shape tests sweep configurations; these sweep shapes."""
 import numpy as np
 import pytest
 
@@ -25,6 +25,12 @@ def uniform(rng: SplitMix64, lo: float, hi: float) -> float:
     return lo + (hi - lo) * rng.below(1 << 20) / float(1 << 20)
 
 
+def first_dm(rng: SplitMix64) -> float:
+    """0 for one shape in four, else a first DM in [0, 60): a nonzero first
+    DM is where a fused multiply-add in the shift rule would show."""
+    return 0.0 if rng.below(4) == 0 else uniform(rng, 0.0, 60.0)
+
+
 def zapped_of(rng: SplitMix64, channels: int):
     n = rng.below(channels // 5 + 1)
     return tuple(sorted(rng.sample(channels, n))) if n else ()
@@ -37,7 +43,8 @@ def single_shape(seed: int):
     return dict(channels=channels, samples=1 + rng.below(3000), dms=1 + rng.below(80),
                 step=uniform(rng, 0.0, 2.0), min_freq=uniform(rng, 1100.0, 1500.0),
                 bandwidth=uniform(rng, 0.2, 1.5), beams=beams, sbeams=beams + rng.below(2),
-                zapped=zapped_of(rng, channels), seed=seed), PADDINGS[rng.below(3)], rng.below(len(SWEEP_CONFS) + 1)
+                zapped=zapped_of(rng, channels), seed=seed, first=first_dm(rng)), \
+        PADDINGS[rng.below(3)], rng.below(len(SWEEP_CONFS) + 1)
 
 
 def two_step_shape(seed: int):
@@ -48,7 +55,8 @@ def two_step_shape(seed: int):
     return dict(channels=channels, samples=1 + rng.below(2000), dms=1 + rng.below(40), step=uniform(rng, 0.0, 1.5),
                 subbands=subbands, coarse=1 + rng.below(12), coarse_step=uniform(rng, 0.0, 8.0),
                 min_freq=uniform(rng, 1100.0, 1500.0), bandwidth=uniform(rng, 0.2, 1.5), beams=beams,
-                sbeams=beams + rng.below(2), zapped=zapped_of(rng, channels), seed=seed), \
+                sbeams=beams + rng.below(2), zapped=zapped_of(rng, channels), seed=seed, first=first_dm(rng),
+                coarse_first=first_dm(rng)), \
         PADDINGS[rng.below(3)], rng.below(len(STEP_ONE_CONFS) + 1), rng.below(len(STEP_TWO_CONFS) + 1)
 
 
@@ -99,7 +107,7 @@ def test_single_step_packed_random_shape(torch_cuda, seed):
     w = workloads.tiny(channels=channels, samples=1 + rng.below(3000), dms=1 + rng.below(40),
                        step=uniform(rng, 0.0, 0.5), min_freq=uniform(rng, 1300.0, 1500.0),
                        bandwidth=uniform(rng, 0.2, 1.0), beams=beams, sbeams=beams + rng.below(2),
-                       zapped=zapped_of(rng, channels), seed=seed)
+                       zapped=zapped_of(rng, channels), seed=seed, first=first_dm(rng) / 4)
     bits = (1, 2, 4)[rng.below(3)]
     padding = PADDINGS[rng.below(3)]
     conf = pick(SWEEP_CONFS, rng.below(len(SWEEP_CONFS) + 1))

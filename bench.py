@@ -48,6 +48,8 @@ def parse():
                     choices=["batched-beams", "arts-two-step", "arts-single", "detect-peaks"])
     ap.add_argument("--input-bits", type=int, default=8, choices=[1, 2, 4, 8],
                     help="bits per input sample (1, 2, 4 = packed rows of uniform random fields)")
+    ap.add_argument("--chunks", type=int, default=None,
+                    help="batch pipeline depth (default: the packaged tuning's; 0 = persistent launches in order)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -282,13 +284,14 @@ def run_batched(args, torch):
     from dedispersion_amd.beams import BeamSet
 
     ws = workloads.batched_beams()
-    bs = BeamSet(torch, ws, input_bits=args.input_bits)
+    bs = BeamSet(torch, ws, input_bits=args.input_bits, chunks=args.chunks)
     info = bs.info()
     units = info["units"]
     log(f"batched beams: {info}")
     for _ in range(args.warmup):
         bs.run(use_graph=True)
         bs.run_per_job()
+        bs.run_per_beam_same_format()
     torch.cuda.synchronize()
 
     # timed region: K graph replays of the whole batch
@@ -315,6 +318,15 @@ def run_batched(args, torch):
             pairs[m].append((a, b))
     torch.cuda.synchronize()
     avg = {m: sum(a.elapsed_time(b) for a, b in v) / len(v) for m, v in pairs.items()}
+
+    # the batch's kernels and uint16 intermediate, one beam at a time
+    a, b = events(torch, 2)
+    a.record()
+    for _ in range(args.steps):
+        bs.run_per_beam_same_format()
+    b.record()
+    torch.cuda.synchronize()
+    ms_same = a.elapsed_time(b) / args.steps
 
     # the reference's launch pattern: one launch per beam and step
     a, b = events(torch, 2)
@@ -364,6 +376,11 @@ def run_batched(args, torch):
                                      "by its DM counts, float intermediate",
                            "launches_per_step": 2 * len(ws)},
         "batched_vs_plain": ms_plain / ms,
+        "per_beam_same_format": {"value": units / (ms_same / 1e3), "ms_per_step": ms_same,
+                                 "launch": "the batch's kernels, configurations and uint16 intermediate, one "
+                                           "persistent launch per beam and step",
+                                 "launches_per_step": 2 * len(ws)},
+        "batched_vs_per_beam_same_format": ms_same / ms,
         "intermediate": "uint16" if info["compact_intermediate"] else "float32",
         "bytes_per_unit": total_bytes / units,
         "hbm_roofline_frac_path": total_bytes / (ms / 1e3) / HBM_PEAK,

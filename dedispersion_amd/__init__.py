@@ -324,6 +324,13 @@ class Batch:
     def run_per_job(self, stream: int = 0) -> None:
         check(self.lib.dd_batch_run_per_job(self.handle, stream or None))
 
+    def job_placement(self, job: int) -> dict:
+        """Where job `job` runs: {"beside": beside the batch kernels, with the
+        float intermediate; "compact": its intermediate holds uint16 sums}."""
+        beside, compact = ctypes.c_uint32(), ctypes.c_uint32()
+        check(self.lib.dd_batch_get_job_placement(self.handle, job, ctypes.byref(beside), ctypes.byref(compact)))
+        return {"beside": bool(beside.value), "compact": bool(compact.value)}
+
     def info(self) -> dict:
         i = _lib.DDBatchInfo()
         check(self.lib.dd_batch_get_info(self.handle, ctypes.byref(i)))

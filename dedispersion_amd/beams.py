@@ -17,7 +17,7 @@ from .workloads import PADDING, Workload
 
 class BeamSet:
     def __init__(self, torch, workloads: List[Workload], padding: int = PADDING, confs: Optional[dict] = None,
-                 pulses: bool = True, input_bits: int = 8, tuned_plans: bool = True):
+                 pulses: bool = True, input_bits: int = 8, tuned_plans: bool = True, chunks: Optional[int] = None):
         """confs: {mode: Conf} for the batch; None = the tuned batch
         configurations packaged with the library (the tuned pipelined pair
         and chunk count when there is one, else the best of each kind, else
@@ -26,7 +26,9 @@ class BeamSet:
         (the workloads' noise model and pulses apply to 8-bit input only).
         tuned_plans: each plan takes its step-one and step-two configurations
         from the packaged tuned-configuration files by its DM counts (the
-        reference's lookup), which the one-launch-per-beam path runs."""
+        reference's lookup), which the one-launch-per-beam path runs.
+        chunks: the batch's pipeline depth (None = the packaged tuning's;
+        0 = one persistent launch per kernel kind, in order)."""
         self.torch = torch
         self.workloads = workloads
         self.padding = padding
@@ -89,6 +91,8 @@ class BeamSet:
         if confs:
             for mode, conf in confs.items():
                 self.batch.set_conf(mode, conf)
+        if chunks is not None:
+            self.batch.set_pipeline(chunks)
         for plan, d_in, d_mid, d_out in zip(self.plans, self.inputs, self.mids, self.outs):
             self.batch.add_two_step(plan, d_in.data_ptr(), d_mid.data_ptr(), d_out.data_ptr())
         torch.cuda.synchronize()
@@ -119,6 +123,25 @@ class BeamSet:
     def run_per_job(self) -> None:
         self.batch.run_per_job(self.stream)
 
+    def run_per_beam_same_format(self) -> None:
+        """The batch's own kernels, configurations and intermediate format,
+        launched one beam at a time: for every beam, step one then step two
+        as persistent launches over that beam's tiles alone (no graph).  The
+        time against the batch separates what batching gains from what the
+        uint16 intermediate gains."""
+        if not hasattr(self, "_single"):
+            self._single = []
+            for plan, d_in, d_mid, d_out in zip(self.plans, self.inputs, self.mids, self.outs):
+                b = Batch()
+                for mode in (DD_STEP_ONE, DD_STEP_TWO):
+                    b.set_conf(mode, self.batch.get_conf(mode))
+                b.set_pipeline(0)
+                b.add_two_step(plan, d_in.data_ptr(), d_mid.data_ptr(), d_out.data_ptr())
+                self._single.append(b)
+        for b in self._single:
+            b.run_mode(DD_STEP_ONE, self.stream)
+            b.run_mode(DD_STEP_TWO, self.stream)
+
     def info(self) -> dict:
         return self.batch.info()
 
@@ -133,6 +156,9 @@ class BeamSet:
         return b[DD_STEP_ONE] + b[DD_STEP_TWO]
 
     def close(self) -> None:
+        for b in getattr(self, "_single", []):
+            b.close()
+        self._single = []
         self.batch.close()
         for p in self.plans:
             p.close()

@@ -81,6 +81,8 @@ struct dd_batch {
   // and the float intermediate
   std::vector<U8Launch> solo_single, solo_one;
   std::vector<F32Launch> solo_two;
+  // per job, in job order: runs beside the batch kernels; its intermediate holds uint16 sums
+  std::vector<bool> beside, compact_job;
   KindQueue q[3];
   DeviceBuffer<uint32_t> heads;  // claim counters, kQueues * kHeadStride per kind
   uint64_t units = 0;
@@ -153,8 +155,10 @@ template <typename L> void merge_geometry(KindQueue &k, const L &launch, bool fi
   }
 }
 
-dd_status build(dd_batch *b) {
-  if (b->built) return DD_OK;
+// The host part of build(): every job's launches, the per-kind geometry,
+// occupancy and tile lists (tiles[m] for kind m); no device memory is
+// touched.
+dd_status plan_batch(dd_batch *b, std::vector<TileDesc> (&tiles_of)[3]) {
   for (auto &k : b->q) {
     k.u8.clear();
     k.f32.clear();
@@ -174,6 +178,8 @@ dd_status build(dd_batch *b) {
   b->solo_single.clear();
   b->solo_one.clear();
   b->solo_two.clear();
+  b->beside.clear();
+  b->compact_job.clear();
   // the batch kernels are the staged ones: a configuration that can never
   // select them is an error; a job whose windows do not fit runs on its own
   for (const Job &job : b->jobs) {
@@ -202,12 +208,14 @@ dd_status build(dd_batch *b) {
       U8Launch L, P;
       s = prepare_single(job.plan, job.plan->conf[DD_SINGLE_STEP], job.in, job.out, job.first_sbeam, job.nr_sbeams,
                          P);
-      if (s == DD_OK)
-        s = prepare_single(job.plan, b->conf[DD_SINGLE_STEP], job.in, job.out, job.first_sbeam, job.nr_sbeams, L);
       if (s != DD_OK) return s;
+      s = prepare_single(job.plan, b->conf[DD_SINGLE_STEP], job.in, job.out, job.first_sbeam, job.nr_sbeams, L);
+      if (s != DD_OK && s != DD_ERR_CONFIGURATION) return s;
       b->pj_single.push_back(P);
       b->units += sbeams * o.nr_dms * z.nr_samples;
-      if (!staged(L)) {
+      b->beside.push_back(s != DD_OK || !staged(L));
+      b->compact_job.push_back(false);
+      if (b->beside.back()) {
         b->solo_single.push_back(P);
         continue;
       }
@@ -232,8 +240,10 @@ dd_status build(dd_batch *b) {
       s = prepare_step_one(job.plan, b->conf[DD_STEP_ONE], job.in, job.mid, L1);
       if (s == DD_OK)
         s = prepare_step_two(job.plan, b->conf[DD_STEP_TWO], job.mid, job.out, job.first_sbeam, job.nr_sbeams, L2);
-      if (s != DD_OK) return s;
-      bool fits = staged(L1);
+      // the plan's own launches prepared above; a batch configuration that
+      // cannot run this job (its LDS, its windows) leaves it beside the batch
+      if (s != DD_OK && s != DD_ERR_CONFIGURATION) return s;
+      bool fits = s == DD_OK && staged(L1);
       if (fits && b->compact) {
         L1.k.out_u16 = 1;  // vector stores of four uint16 sums need 8-byte aligned rows
         L1.k.vec_store = (z.sub_out_pitch % 4 == 0) && (reinterpret_cast<uintptr_t>(job.mid) & 7) == 0;
@@ -246,6 +256,8 @@ dd_status build(dd_batch *b) {
       } else if (fits) {
         fits = staged(L2);
       }
+      b->beside.push_back(!fits);
+      b->compact_job.push_back(fits && b->compact);
       if (!fits) {
         b->solo_one.push_back(P1);
         b->solo_two.push_back(P2);
@@ -262,7 +274,8 @@ dd_status build(dd_batch *b) {
   }
   for (int m = 0; m < 3; ++m) {
     KindQueue &k = b->q[m];
-    std::vector<TileDesc> tiles;
+    std::vector<TileDesc> &tiles = tiles_of[m];
+    tiles.clear();
     const size_t njobs = m == DD_STEP_TWO ? k.f32.size() : k.u8.size();
     for (size_t j = 0; j < njobs; ++j) {
       const uint32_t n = m == DD_STEP_TWO ? k.f32[j].k.n_tiles : k.u8[j].k.n_tiles;
@@ -276,21 +289,36 @@ dd_status build(dd_batch *b) {
       return fail(DD_ERR_CONFIGURATION, "batch LDS demand exceeds the device limit");
     int per_cu = 0;
     if (m == DD_STEP_TWO) {
-      std::vector<RowSumF32Params> params;
-      for (auto &L : k.f32) params.push_back(L.k);
-      DD_HIP(k.d_f32.upload(params));
       if (b->compact) DD_HIP(rowsum_u16_batch_occupancy(k.g, &per_cu));
       else DD_HIP(rowsum_f32_batch_occupancy(k.g, &per_cu));
     } else {
-      std::vector<RowSumU8Params> params;
-      for (auto &L : k.u8) params.push_back(L.k);
-      DD_HIP(k.d_u8.upload(params));
       DD_HIP(rowsum_u8_batch_occupancy(k.g, k.flush, &per_cu));
     }
     if (per_cu < 1) return fail(DD_ERR_CONFIGURATION, "batch kernel cannot be resident on a compute unit");
     const uint32_t resident = (uint32_t)per_cu * (uint32_t)b->cus;
     k.workgroups = std::min(resident, roundup(k.n_tiles, kQueues));
-    DD_HIP(k.d_tiles.upload(tiles));
+  }
+  return DD_OK;
+}
+
+dd_status build(dd_batch *b) {
+  if (b->built) return DD_OK;
+  std::vector<TileDesc> tiles_of[3];
+  dd_status planned = plan_batch(b, tiles_of);
+  if (planned != DD_OK) return planned;
+  for (int m = 0; m < 3; ++m) {
+    KindQueue &k = b->q[m];
+    if (k.n_tiles == 0) continue;
+    if (m == DD_STEP_TWO) {
+      std::vector<RowSumF32Params> params;
+      for (auto &L : k.f32) params.push_back(L.k);
+      DD_HIP(k.d_f32.upload(params));
+    } else {
+      std::vector<RowSumU8Params> params;
+      for (auto &L : k.u8) params.push_back(L.k);
+      DD_HIP(k.d_u8.upload(params));
+    }
+    DD_HIP(k.d_tiles.upload(tiles_of[m]));
     DD_HIP(k.d_begin.upload(queue_bounds(k.n_tiles)));
   }
   if (b->heads.count == 0) DD_HIP(b->heads.allocate(3 * kQueues * kHeadStride));
@@ -535,6 +563,16 @@ dd_status dd_batch_set_compact_intermediate(dd_batch *batch, int enable) {
   if (!batch) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
   batch->compact_requested = enable != 0;
   batch->invalidate();
+  return DD_OK;
+}
+
+dd_status dd_batch_get_job_placement(dd_batch *batch, uint32_t job, uint32_t *beside, uint32_t *compact) {
+  if (!batch || !beside || !compact) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  if (job >= batch->jobs.size()) return fail(DD_ERR_INVALID_ARGUMENT, "job index out of range");
+  dd_status s = build(batch);
+  if (s != DD_OK) return s;
+  *beside = batch->beside[job] ? 1u : 0u;
+  *compact = batch->compact_job[job] ? 1u : 0u;
   return DD_OK;
 }
 

@@ -11,12 +11,18 @@ the best-predicted fraction.  The exhaustive sweep stays the fallback (no
 model, or a budget of 1) and the yardstick (tests compare with it).
 
 Features of a (layout, configuration) pair:
-  threads_d0, log2 threads_d1, log2 items_d0, log2 items_d1, log2 unroll,
-  local (0 cache / 1 pipelined LDS / 2 LDS), log2 rows summed per output
+  log2(1 + the shift spread of a DM tile), log2 threads_d1, log2 items_d0,
+  log2 items_d1, log2 unroll, local (0 cache / 1 pipelined LDS / 2 LDS),
+  log2 rows summed per output
   ("channels"), log2 independent row groups per launch ("subbands"),
   log2 DMs and log2 samples -- both counted in tiles of the configuration
   (ceil(DMs / (threads_d1 items_d1)), ceil(samples / (64 items_d0))), which
   is how the grid sees them --, log2 DM step and log2(1 + largest shift).
+The first slot would hold threads_d0, which is 64 (one wavefront along the
+samples) in every configuration of these kernels and so carries nothing; it
+holds instead the spread of the shifts across one DM tile of the
+configuration, largest shift x DMs per tile / DMs, which sets how much wider
+than the sample tile every staged row window is.
 
 Target: log(run time / additions of the launch), the time per addition, so
 that what is learned on one problem size carries to another; the ranking
@@ -37,7 +43,7 @@ import numpy as np
 
 from ._lib import check, load
 
-FEATURES = ("threads_d0", "log2_threads_d1", "log2_items_d0", "log2_items_d1", "log2_unroll", "local",
+FEATURES = ("log2_1p_tile_spread", "log2_threads_d1", "log2_items_d0", "log2_items_d1", "log2_unroll", "local",
             "log2_rows", "log2_groups", "log2_dm_tiles", "log2_sample_tiles", "log2_dm_step", "log2_1p_max_shift")
 CONF_KEYS = ("threads_d0", "threads_d1", "items_d0", "items_d1", "unroll", "local")
 RECORDS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "sweep_mi355x.jsonl.gz")
@@ -47,8 +53,9 @@ def feature_row(layout: Dict, conf: Dict) -> List[float]:
     mode = layout["mode"]
     rows = layout["rows"] / layout["groups"] if mode == "step_one" else layout["rows"]
     groups = layout["beams"] * layout["groups"]
+    tile_dms = conf["threads_d1"] * conf["items_d1"]
     return [
-        conf["threads_d0"] / 64.0,
+        math.log2(1.0 + layout["max_shift"] * min(tile_dms, layout["dms"]) / max(layout["dms"], 1)),
         math.log2(conf["threads_d1"]),
         math.log2(conf["items_d0"]),
         math.log2(conf["items_d1"]),

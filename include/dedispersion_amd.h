@@ -381,6 +381,12 @@ dd_status dd_batch_run_mode(dd_batch *batch, dd_mode mode, void *stream);
  * configuration and the float intermediate of the reference layout. */
 dd_status dd_batch_run_per_job(dd_batch *batch, void *stream);
 dd_status dd_batch_get_info(dd_batch *batch, dd_batch_info *info);
+/* Where job `job` (in the order of the dd_batch_add_* calls) runs: beside = 1
+ * when it runs beside the batch kernels as its plan's own launches (its
+ * intermediate then holds the reference's floats); compact = 1 when its
+ * intermediate holds the uint16 sums of the compact intermediate (a
+ * two-step job in the batch kernels with compact intermediate in effect). */
+dd_status dd_batch_get_job_placement(dd_batch *batch, uint32_t job, uint32_t *beside, uint32_t *compact);
 
 /* ---- streaming: continuous two-step dedispersion of chunked beams ----
  *

@@ -336,6 +336,9 @@ def test_batch_runs_wide_jobs_beside_the_kernels(torch_cuda):
             b.set_pipeline(chunks)
             info = b.info()
             assert info["solo_jobs"] == 1 and info["compact_intermediate"] == 1
+            assert [b.job_placement(j) for j in range(3)] == [
+                {"beside": False, "compact": True}, {"beside": False, "compact": True},
+                {"beside": True, "compact": False}]
             want_units = sum(w.obs.nr_synthesized_beams * w.obs.nr_dms_subbanding * w.obs.nr_dms *
                              w.obs.nr_samples_per_batch for w in ws)
             assert info["units"] == want_units
@@ -346,6 +349,33 @@ def test_batch_runs_wide_jobs_beside_the_kernels(torch_cuda):
         b.run_mode(DD_STEP_ONE, stream=stream_of(torch_cuda))
         b.run_mode(DD_STEP_TWO, stream=stream_of(torch_cuda))
         jobs.check("per kind", compact=compact)
+
+
+def test_batch_configuration_whose_float_windows_overflow_lds(torch_cuda):
+    """A step-two configuration of 8 wavefronts x 16 samples and unroll 16
+    over a job whose step-two shifts spread about 200 samples across a
+    64-DM tile: 16 float row windows per buffer would not fit in LDS.  The
+    batch does not fail: the staged kernels stage fewer rows per barrier (or
+    the job runs beside them), with the float intermediate and with uint16,
+    and the results equal the oracle's."""
+    w = workloads.tiny(channels=256, samples=1500, dms=64, step=0.05, subbands=16, coarse=4, coarse_step=0.5,
+                       min_freq=600.0, bandwidth=1.0, seed=38)
+    orc = oracle.Oracle(w.obs, 0)
+    s2 = orc.get_shifts_step_two()
+    spread = orc.shift(0.0, 0.05, 63, float(s2[0])) - orc.shift(0.0, 0.05, 0, float(s2[0]))
+    assert 150 <= spread <= 260, spread
+    conf_two = Conf(64, 8, 16, 8, 16, 1)
+    for compact in (False, True):
+        jobs = Jobs(torch_cuda, [w, tiny_two_step_workloads()[1]])
+        with Batch() as b:
+            b.set_conf(DD_STEP_ONE, STAGED_SMALL)
+            b.set_conf(DD_STEP_TWO, conf_two)
+            b.set_compact_intermediate(compact)
+            for plan, d_in, d_mid, d_out in zip(jobs.plans, jobs.inputs, jobs.mids, jobs.outs):
+                b.add_two_step(plan, d_in.data_ptr(), d_mid.data_ptr(), d_out.data_ptr())
+            place = [b.job_placement(j) for j in range(2)]
+            b.run(stream=stream_of(torch_cuda))
+            jobs.check(f"wide float windows, compact {compact}", compact=[p["compact"] for p in place])
 
 
 def test_batch_runs_wide_single_step_jobs_beside_the_kernels(torch_cuda):

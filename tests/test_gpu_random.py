@@ -65,24 +65,6 @@ def pick(confs, i):
     return None if i == len(confs) else confs[i]
 
 
-def u16_window_fits(w, conf: Conf) -> bool:
-    """Whether a batch's uint16 step two can stage this job's rows: a row
-    window is the sample tile (64 items_d0) plus the largest spread of the
-    step-two shifts over a tile of threads_d1 x items_d1 DMs (a tile past the
-    last DM ends at the last DM), plus 8 elements of alignment slack, and a
-    lane stages at most items_d0 / 8 + 1 loads of 8 elements.  Shifts from
-    the oracle."""
-    o = w.obs
-    orc = oracle.Oracle(o, 0, threads=1)
-    table = orc.get_shifts_step_two()
-    tile = conf.threads_d1 * conf.items_d1
-    spread = max(orc.shift(o.first_dm, o.dm_step, min(d0 + tile, o.nr_dms) - 1, table[sb]) -
-                 orc.shift(o.first_dm, o.dm_step, d0, table[sb])
-                 for sb in range(o.nr_subbands) for d0 in range(0, o.nr_dms, tile))
-    win = (64 * conf.items_d0 + spread + 8 + 7) // 8 * 8
-    return win // 8 <= 64 * (conf.items_d0 // 8 + 1)
-
-
 @pytest.mark.parametrize("seed", range(24))
 def test_single_step_random_shape(torch_cuda, seed):
     shape, padding, ci = single_shape(seed)
@@ -160,7 +142,6 @@ def test_batch_random_jobs(torch_cuda, seed):
         return Conf(64, t1, (8, 16)[rng.below(2)], (2, 4, 8)[rng.below(3)], 1 + rng.below(min(4, 2 * t1)), 1)
 
     one, two = staged_conf(), staged_conf()
-    fits = [u16_window_fits(w, two) for w in ws]
     jobs = Jobs(torch_cuda, ws)
     what = f"seed {seed} chunks {chunks} confs {one} / {two}"
     with Batch() as b:
@@ -171,12 +152,15 @@ def test_batch_random_jobs(torch_cuda, seed):
             b.add_two_step(plan, d_in.data_ptr(), d_mid.data_ptr(), d_out.data_ptr())
         info = b.info()
         assert info["compact_intermediate"] == 1
-        # a job too wide for the uint16 step-two window runs beside the batch
-        # kernels with its plan's own configuration and the float intermediate
-        assert info["solo_jobs"] == fits.count(False), what
+        # a job too wide for the batch kernels' windows runs beside them with
+        # its plan's own configuration and the float intermediate
+        place = [b.job_placement(j) for j in range(len(ws))]
+        assert info["solo_jobs"] == sum(p["beside"] for p in place), what
+        assert all(p["compact"] != p["beside"] for p in place), what
+        compact = [p["compact"] for p in place]
         for use_graph in (False, True):
             b.run(use_graph=use_graph, stream=stream_of(torch_cuda))
-            jobs.check(f"batch (graph {use_graph}), " + what, compact=fits)
+            jobs.check(f"batch (graph {use_graph}), " + what, compact=compact)
             jobs.reset()
         b.run_per_job(stream=stream_of(torch_cuda))
         jobs.check("per job, " + what)

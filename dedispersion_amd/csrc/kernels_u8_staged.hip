@@ -22,8 +22,11 @@
 // include/Dedispersion.hpp:92-103) are loaded as packed bytes and unpacked
 // to one byte per sample in registers before the copies are built.
 //
-// The summation adds v_perm_b32-extracted 16-bit pairs, as kernels_u8.hip
-// does; partial sums are exact integers.  Each half-wavefront sums one DM at
+// The summation keeps two 32-bit accumulators per four-sample group: `all`,
+// the plain 32-bit sum of the groups' dwords (bytes carry into their
+// neighbours), and `odd`, the odd bytes extracted by v_perm_b32 into 16-bit
+// pairs.  all - (odd << 8) is then the even bytes' 16-bit pairs, exactly, as
+// long as every pair sum stays below 2^16; partial sums are exact integers.  Each half-wavefront sums one DM at
 // a time (lane l: DM 2s + l / 32 of the wavefront's DMs, over the groups
 // (l % 32) + 32q), so every lane reads its own DM's LDS address.
 #include "staged_common.hpp"
@@ -247,29 +250,34 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
   };
 
   // ---- accumulation ----
-  uint32_t even[DPW][NG], odd[DPW][NG];
+  // all[j][k]: 32-bit sum of the group dwords, B0 + 2^8 B1 + 2^16 B2 + 2^24 B3
+  // (mod 2^32) for byte sums B0..B3; odd[j][k] = B1 + 2^16 B3.  Then
+  // all - (odd << 8) = B0 + 2^16 B2 exactly while every B stays below 2^16.
+  uint32_t all[DPW][NG], odd[DPW][NG];
   uint32_t wide[DPW][RUN];
 #pragma unroll
   for (int j = 0; j < DPW; ++j)
 #pragma unroll
-    for (int k = 0; k < NG; ++k) even[j][k] = odd[j][k] = 0;
+    for (int k = 0; k < NG; ++k) all[j][k] = odd[j][k] = 0;
   if constexpr (FLUSH) {
 #pragma unroll
     for (int j = 0; j < DPW; ++j)
 #pragma unroll
       for (int k = 0; k < RUN; ++k) wide[j][k] = 0;
   }
+  auto even_of = [&](int j, int k) -> uint32_t { return all[j][k] - (odd[j][k] << 8); };
   auto flush = [&]() {
     if constexpr (FLUSH) {
 #pragma unroll
       for (int j = 0; j < DPW; ++j)
 #pragma unroll
         for (int k = 0; k < NG; ++k) {
-          wide[j][4 * k + 0] += even[j][k] & 0xFFFFu;
-          wide[j][4 * k + 2] += even[j][k] >> 16;
+          const uint32_t even = even_of(j, k);
+          wide[j][4 * k + 0] += even & 0xFFFFu;
+          wide[j][4 * k + 2] += even >> 16;
           wide[j][4 * k + 1] += odd[j][k] & 0xFFFFu;
           wide[j][4 * k + 3] += odd[j][k] >> 16;
-          even[j][k] = odd[j][k] = 0;
+          all[j][k] = odd[j][k] = 0;
         }
     }
   };
@@ -284,9 +292,9 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
   // dwords (no bank conflicts), and each lane reads its DM's word itself
   // (no v_readlane).  Chunk c (NG groups) of the lane is accumulator [c][k],
   // slot c / CPS, groups (c % CPS) NG + k.  A group is one aligned dword of
-  // copy (offset & 3); even and odd samples go to 16-bit pairs with constant
-  // v_perm selectors.
-  constexpr uint32_t SEL_EVEN = 0x0c020c00u, SEL_ODD = 0x0c030c01u;
+  // copy (offset & 3); it is added whole to `all`, and its odd samples,
+  // extracted to 16-bit pairs with a constant v_perm selector, to `odd`.
+  constexpr uint32_t SEL_ODD = 0x0c030c01u;
   constexpr int LPD = DPW == 1 ? 64 : 32;  // lanes per DM
   constexpr int CPS = 64 / LPD;            // DMs per slot = chunks per slot
   constexpr int SL = DPW / CPS;            // DM slots per lane
@@ -321,7 +329,7 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
         const uint32_t(&x1)[NG] = x[c & 1][1];
 #pragma unroll
         for (int k = 0; k < NG; ++k) {
-          even[c][k] += __builtin_amdgcn_perm(x0[k], x0[k], SEL_EVEN) + __builtin_amdgcn_perm(x1[k], x1[k], SEL_EVEN);
+          all[c][k] += x0[k] + x1[k];
           odd[c][k] += __builtin_amdgcn_perm(x0[k], x0[k], SEL_ODD) + __builtin_amdgcn_perm(x1[k], x1[k], SEL_ODD);
         }
       }
@@ -336,7 +344,7 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
 #pragma unroll
           for (int k = 0; k < NG; ++k) {
             const uint32_t v = src[LPD * (cc * NG + k)];
-            even[c][k] += __builtin_amdgcn_perm(v, v, SEL_EVEN);
+            all[c][k] += v;
             odd[c][k] += __builtin_amdgcn_perm(v, v, SEL_ODD);
           }
         }
@@ -391,8 +399,9 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
           w0 = (wide[j][4 * k] & 0xFFFFu) | (wide[j][4 * k + 1] << 16);
           w1 = (wide[j][4 * k + 2] & 0xFFFFu) | (wide[j][4 * k + 3] << 16);
         } else {
-          w0 = __builtin_amdgcn_perm(odd[j][k], even[j][k], 0x05040100u);
-          w1 = __builtin_amdgcn_perm(odd[j][k], even[j][k], 0x07060302u);
+          const uint32_t even = even_of(j, k);
+          w0 = __builtin_amdgcn_perm(odd[j][k], even, 0x05040100u);
+          w1 = __builtin_amdgcn_perm(odd[j][k], even, 0x07060302u);
         }
         const uint32_t t = t_of(j, k);
         if (p.vec_store && t + 3 < p.n_samples) {
@@ -422,9 +431,10 @@ __device__ __forceinline__ void u8_tile(const RowSumU8Params &p, uint32_t logica
 #pragma unroll
         for (int e = 0; e < 4; ++e) v[e] = (float)wide[j][4 * k + e];
       } else {
-        v[0] = (float)(even[j][k] & 0xFFFFu);
+        const uint32_t even = even_of(j, k);
+        v[0] = (float)(even & 0xFFFFu);
         v[1] = (float)(odd[j][k] & 0xFFFFu);
-        v[2] = (float)(even[j][k] >> 16);
+        v[2] = (float)(even >> 16);
         v[3] = (float)(odd[j][k] >> 16);
       }
       const uint32_t t = t_of(j, k);

@@ -182,6 +182,7 @@ PROTOTYPES = {
     "dd_batch_run_mode": (c_int, [c_void_p, c_int, c_void_p]),
     "dd_batch_run_per_job": (c_int, [c_void_p, c_void_p]),
     "dd_batch_get_info": (c_int, [c_void_p, POINTER(DDBatchInfo)]),
+    "dd_batch_run_static": (c_int, [c_void_p, c_void_p]),
     "dd_batch_get_job_placement": (c_int, [c_void_p, c_uint32, POINTER(c_uint32), POINTER(c_uint32)]),
     "dd_stream_create": (c_int, [POINTER(c_void_p), c_void_p, c_uint32, c_uint64, c_uint32, c_uint32]),
     "dd_stream_destroy": (c_int, [c_void_p]),

@@ -32,10 +32,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <string>
 #include <vector>
 
-#include "plan_internal.hpp"
+#include "batch_internal.hpp"
 
 using namespace ddamd;
 
@@ -97,6 +98,8 @@ struct dd_batch {
   DeviceBuffer<TileDesc> d_list1, d_list2;     // step one / step two tiles, chunk after chunk
   hipStream_t side[2] = {nullptr, nullptr};    // step one, step two
   std::vector<hipEvent_t> events;              // fork, join x2, one per chunk
+  // the allocation-free run: the host image of its last device area
+  ddamd::StaticBatchImage static_image;
 
   void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -448,6 +451,87 @@ dd_status add_job(dd_batch *b, const Job &job) {
 }
 
 } // namespace
+
+namespace ddamd {
+
+size_t batch_job_count(const dd_batch *b) { return b->jobs.size(); }
+
+StaticBatchImage &batch_image(dd_batch *b) { return b->static_image; }
+
+// Lays out the persistent launches of a batch -- claim counters, then per
+// kernel kind its parameter blocks, tile descriptors and queue bounds, each
+// at a 256-byte aligned offset -- as one host image for a device area of
+// `capacity` bytes.  Jobs that would run beside the batch kernels are
+// refused: their plans' own launches are not part of the image.
+dd_status batch_static_image(dd_batch *b, uint64_t capacity, StaticBatchImage &img) {
+  std::vector<TileDesc> tiles_of[3];
+  dd_status st = plan_batch(b, tiles_of);
+  if (st != DD_OK) return st;
+  if (!b->solo_single.empty() || !b->solo_one.empty())
+    return fail(DD_ERR_UNSUPPORTED, "a job's row windows do not fit the batch kernels; the allocation-free run "
+                                    "takes only jobs that the batch kernels run");
+  auto align = [](uint64_t v) { return (v + 255) & ~uint64_t(255); };
+  uint64_t at = align(3ull * kQueues * kHeadStride * sizeof(uint32_t));
+  img.heads = 0;
+  for (int m = 0; m < 3; ++m) {
+    const KindQueue &k = b->q[m];
+    img.params[m] = img.tiles[m] = img.begin[m] = 0;
+    if (k.n_tiles == 0) continue;
+    const uint64_t pbytes = m == DD_STEP_TWO ? k.f32.size() * sizeof(RowSumF32Params)
+                                             : k.u8.size() * sizeof(RowSumU8Params);
+    img.params[m] = at;
+    at = align(at + pbytes);
+    img.tiles[m] = at;
+    at = align(at + (uint64_t)k.n_tiles * sizeof(TileDesc));
+    img.begin[m] = at;
+    at = align(at + (kQueues + 1) * sizeof(uint32_t));
+  }
+  img.bytes = at;
+  if (img.bytes > capacity)
+    return fail(DD_ERR_UNSUPPORTED, "the batch's work queues need " + std::to_string(img.bytes) +
+                                        " bytes; the allocation-free run holds at most " + std::to_string(capacity));
+  img.host.assign(img.bytes, 0);
+  for (int m = 0; m < 3; ++m) {
+    const KindQueue &k = b->q[m];
+    if (k.n_tiles == 0) continue;
+    uint8_t *base = img.host.data();
+    if (m == DD_STEP_TWO)
+      for (size_t j = 0; j < k.f32.size(); ++j)
+        std::memcpy(base + img.params[m] + j * sizeof(RowSumF32Params), &k.f32[j].k, sizeof(RowSumF32Params));
+    else
+      for (size_t j = 0; j < k.u8.size(); ++j)
+        std::memcpy(base + img.params[m] + j * sizeof(RowSumU8Params), &k.u8[j].k, sizeof(RowSumU8Params));
+    std::memcpy(base + img.tiles[m], tiles_of[m].data(), tiles_of[m].size() * sizeof(TileDesc));
+    const std::vector<uint32_t> bounds = queue_bounds(k.n_tiles);
+    std::memcpy(base + img.begin[m], bounds.data(), bounds.size() * sizeof(uint32_t));
+  }
+  return DD_OK;
+}
+
+// The persistent launches of the image at device address `dev` (copied
+// there beforehand): per kernel kind, its claim counters cleared by a kernel,
+// then its persistent kernel, on stream s.
+dd_status batch_enqueue_static(dd_batch *b, uint8_t *dev, const StaticBatchImage &img, hipStream_t s) {
+  for (int m = 0; m < 3; ++m) {
+    const KindQueue &k = b->q[m];
+    if (k.n_tiles == 0) continue;
+    uint32_t *heads = reinterpret_cast<uint32_t *>(dev + img.heads) + m * kQueues * kHeadStride;
+    DD_HIP(launch_clear_words(heads, kQueues * kHeadStride, s));
+    TileQueues tq{reinterpret_cast<const TileDesc *>(dev + img.tiles[m]),
+                  reinterpret_cast<const uint32_t *>(dev + img.begin[m]), heads};
+    if (m == DD_STEP_TWO) {
+      const auto *params = reinterpret_cast<const RowSumF32Params *>(dev + img.params[m]);
+      if (b->compact) DD_HIP(launch_rowsum_u16_batch(params, tq, k.g, k.workgroups, s));
+      else DD_HIP(launch_rowsum_f32_batch(params, tq, k.g, k.workgroups, s));
+    } else {
+      const auto *params = reinterpret_cast<const RowSumU8Params *>(dev + img.params[m]);
+      DD_HIP(launch_rowsum_u8_batch(params, tq, k.g, k.flush, k.workgroups, s));
+    }
+  }
+  return DD_OK;
+}
+
+} // namespace ddamd
 
 extern "C" {
 

@@ -388,6 +388,25 @@ dd_status dd_batch_get_info(dd_batch *batch, dd_batch_info *info);
  * two-step job in the batch kernels with compact intermediate in effect). */
 dd_status dd_batch_get_job_placement(dd_batch *batch, uint32_t job, uint32_t *beside, uint32_t *compact);
 
+/* Allocation-free run of a batch.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  It enqueues on
+ * `stream` the persistent launches of dd_batch_run_mode for every kernel
+ * kind in order (single step, step one, step two) -- bit-identical results
+ * -- with the work queues (job parameter blocks, tile lists, queue bounds,
+ * claim counters) in a fixed device array of DD_BATCH_STATIC_BYTES per GPU
+ * instead of buffers the batch allocates; it calls no device allocator.  The
+ * queues are copied there from host memory on `stream` and the claim
+ * counters are cleared by a kernel, so consecutive calls in one stream see
+ * only their own queues; every call on a GPU shares that GPU's array, so
+ * calls that overlap on one GPU (from several streams or threads) race.
+ * Fails with DD_ERR_UNSUPPORTED when the queues need more than
+ * DD_BATCH_STATIC_BYTES (the message gives the size) or when a job would run
+ * beside the batch kernels (see dd_batch_set_conf).  The plans, their tables
+ * and the job buffers are the caller's, set up beforehand. */
+#define DD_BATCH_STATIC_BYTES (4u << 20)
+dd_status dd_batch_run_static(dd_batch *batch, void *stream);
+
 /* ---- streaming: continuous two-step dedispersion of chunked beams ----
  *
  * No counterpart in the reference (its driver processes one batch,

@@ -81,15 +81,18 @@ def test_no_register_indexing_and_no_scalar_cache_writes(assembly):
 
 def test_no_scratch_outside_the_packed_flush_kernels(assembly):
     spilling = []
+    total = 0
     for src, asm in assembly.items():
-        ks = kernels(asm)
-        assert ks, src
+        # a source may hold device data only (the allocation-free run's array)
+        ks = kernels(asm) if "amdhsa.kernels:" in asm else []
+        total += len(ks)
         for name, scratch, _ in ks:
             if scratch == 0:
                 continue
             assert SPILL_ALLOWED.search(name) and "rowsum_u8" in name, f"{src}: {name} uses {scratch} bytes of scratch"
             assert scratch <= SPILL_LIMIT, f"{src}: {name} spills {scratch} bytes"
             spilling.append(name)
+    assert total >= 100
     # the allowed spills are what they are documented to be: at most the
     # three forms (per beam, persistent, tile list) of the one shape
     assert len(spilling) <= 3, spilling

@@ -315,6 +315,9 @@ class Batch:
     def set_compact_intermediate(self, enable: bool) -> None:
         check(self.lib.dd_batch_set_compact_intermediate(self.handle, 1 if enable else 0))
 
+    def set_paired_step_two(self, enable: bool) -> None:
+        check(self.lib.dd_batch_set_paired_step_two(self.handle, 1 if enable else 0))
+
     def run(self, use_graph: bool = False, stream: int = 0) -> None:
         check(self.lib.dd_batch_run(self.handle, 1 if use_graph else 0, stream or None))
 
@@ -336,7 +339,8 @@ class Batch:
         check(self.lib.dd_batch_get_info(self.handle, ctypes.byref(i)))
         return {"jobs": i.jobs, "tiles": list(i.tiles), "workgroups": list(i.workgroups),
                 "lds_bytes": list(i.lds_bytes), "units": i.units, "compact_intermediate": i.compact_intermediate,
-                "pipeline_chunks": i.pipeline_chunks, "solo_jobs": i.solo_jobs}
+                "pipeline_chunks": i.pipeline_chunks, "solo_jobs": i.solo_jobs,
+                "paired_step_two": i.paired_step_two}
 
 
 def synth_gaussian_u8(d_data: int, rows: int, cols: int, pitch: int, mean: float, std: float, seed: int,

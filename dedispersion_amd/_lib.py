@@ -110,6 +110,7 @@ class DDBatchInfo(ctypes.Structure):
         ("compact_intermediate", c_uint32),
         ("pipeline_chunks", c_uint32),
         ("solo_jobs", c_uint32),
+        ("paired_step_two", c_uint32),
     ]
 
 
@@ -183,6 +184,7 @@ PROTOTYPES = {
     "dd_batch_run_per_job": (c_int, [c_void_p, c_void_p]),
     "dd_batch_get_info": (c_int, [c_void_p, POINTER(DDBatchInfo)]),
     "dd_batch_run_static": (c_int, [c_void_p, c_void_p]),
+    "dd_batch_set_paired_step_two": (c_int, [c_void_p, c_int]),
     "dd_batch_get_job_placement": (c_int, [c_void_p, c_uint32, POINTER(c_uint32), POINTER(c_uint32)]),
     "dd_stream_create": (c_int, [POINTER(c_void_p), c_void_p, c_uint32, c_uint64, c_uint32, c_uint32]),
     "dd_stream_destroy": (c_int, [c_void_p]),

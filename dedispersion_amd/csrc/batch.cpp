@@ -73,6 +73,7 @@ struct dd_batch {
   dd_conf conf[3]{};
   bool compact_requested = true;
   bool compact = false;  // in effect after build()
+  bool paired_requested = false;  // the compact step two adds subbands in pairs when every job allows it
   bool built = false;
   // one ordinary launch per job (float intermediate): dd_batch_run_per_job
   std::vector<U8Launch> pj_single, pj_one;
@@ -158,10 +159,10 @@ template <typename L> void merge_geometry(KindQueue &k, const L &launch, bool fi
   }
 }
 
-// The host part of build(): every job's launches, the per-kind geometry,
-// occupancy and tile lists (tiles[m] for kind m); no device memory is
-// touched.
-dd_status plan_batch(dd_batch *b, std::vector<TileDesc> (&tiles_of)[3]) {
+// Every job's launches under the batch configuration (and its plan's own),
+// sorted into the kind queues or beside them; `pairs`: prepare the compact
+// step two's paired kernel where a job allows it.
+dd_status plan_jobs(dd_batch *b, bool pairs) {
   for (auto &k : b->q) {
     k.u8.clear();
     k.f32.clear();
@@ -170,11 +171,6 @@ dd_status plan_batch(dd_batch *b, std::vector<TileDesc> (&tiles_of)[3]) {
     k.flush = false;
   }
   b->units = 0;
-  if (b->cus == 0) {
-    int dev = 0;
-    DD_HIP(hipGetDevice(&dev));
-    DD_HIP(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev));
-  }
   b->pj_single.clear();
   b->pj_one.clear();
   b->pj_two.clear();
@@ -183,23 +179,6 @@ dd_status plan_batch(dd_batch *b, std::vector<TileDesc> (&tiles_of)[3]) {
   b->solo_two.clear();
   b->beside.clear();
   b->compact_job.clear();
-  // the batch kernels are the staged ones: a configuration that can never
-  // select them is an error; a job whose windows do not fit runs on its own
-  for (const Job &job : b->jobs) {
-    const int m0 = job.kind == kSingleStep ? DD_SINGLE_STEP : DD_STEP_ONE;
-    const int m1 = job.kind == kSingleStep ? DD_SINGLE_STEP : DD_STEP_TWO;
-    for (int m = m0; m <= m1; ++m) {
-      const dd_conf &c = b->conf[m];
-      if (c.local != 1 || c.unroll > kStagedSlotsPerWave * c.threads_d1)
-        return fail(DD_ERR_CONFIGURATION, "batch configuration of mode " + std::to_string(m) +
-                                              " does not select the staged pipeline (needs local = 1 and unroll <= 2 "
-                                              "* threads_d1)");
-    }
-  }
-  // compact intermediate: every two-step job's subband sums fit 16 bits
-  b->compact = b->compact_requested;
-  for (const Job &job : b->jobs)
-    if (job.kind == kTwoStep && 255u * std::max(job.plan->max_live_sub, 1u) > 65535u) b->compact = false;
   for (size_t j = 0; j < b->jobs.size(); ++j) {
     const Job &job = b->jobs[j];
     const dd_observation &o = job.plan->obs;
@@ -253,7 +232,7 @@ dd_status plan_batch(dd_batch *b, std::vector<TileDesc> (&tiles_of)[3]) {
         // the configuration is staged and its float form prepared: what fails
         // here is the uint16 row window or its LDS
         s = prepare_step_two_u16(job.plan, b->conf[DD_STEP_TWO], job.mid, job.out, job.first_sbeam, job.nr_sbeams,
-                                 L2);
+                                 L2, pairs);
         if (s == DD_ERR_CONFIGURATION) fits = false;
         else if (s != DD_OK) return s;
       } else if (fits) {
@@ -275,6 +254,48 @@ dd_status plan_batch(dd_batch *b, std::vector<TileDesc> (&tiles_of)[3]) {
       k2.f32.push_back(L2);
     }
   }
+  return DD_OK;
+}
+
+// The host part of build(): every job's launches, the per-kind geometry,
+// occupancy and tile lists (tiles[m] for kind m); no device memory is
+// touched.
+dd_status plan_batch(dd_batch *b, std::vector<TileDesc> (&tiles_of)[3]) {
+  if (b->cus == 0) {
+    int dev = 0;
+    DD_HIP(hipGetDevice(&dev));
+    DD_HIP(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev));
+  }
+  // the batch kernels are the staged ones: a configuration that can never
+  // select them is an error; a job whose windows do not fit runs on its own
+  for (const Job &job : b->jobs) {
+    const int m0 = job.kind == kSingleStep ? DD_SINGLE_STEP : DD_STEP_ONE;
+    const int m1 = job.kind == kSingleStep ? DD_SINGLE_STEP : DD_STEP_TWO;
+    for (int m = m0; m <= m1; ++m) {
+      const dd_conf &c = b->conf[m];
+      if (c.local != 1 || c.unroll > kStagedSlotsPerWave * c.threads_d1)
+        return fail(DD_ERR_CONFIGURATION, "batch configuration of mode " + std::to_string(m) +
+                                              " does not select the staged pipeline (needs local = 1 and unroll <= 2 "
+                                              "* threads_d1)");
+    }
+  }
+  // compact intermediate: every two-step job's subband sums fit 16 bits
+  b->compact = b->compact_requested;
+  for (const Job &job : b->jobs)
+    if (job.kind == kTwoStep && 255u * std::max(job.plan->max_live_sub, 1u) > 65535u) b->compact = false;
+  // one step-two kernel for the whole batch: the paired one only when every
+  // job in the batch kernels took it
+  bool pairs = b->compact && b->paired_requested;
+  dd_status st = plan_jobs(b, pairs);
+  if (st != DD_OK) return st;
+  if (pairs) {
+    bool all = true;
+    for (const F32Launch &L : b->q[DD_STEP_TWO].f32) all = all && L.g.pairs;
+    if (!all) {
+      st = plan_jobs(b, false);
+      if (st != DD_OK) return st;
+    }
+  }
   for (int m = 0; m < 3; ++m) {
     KindQueue &k = b->q[m];
     std::vector<TileDesc> &tiles = tiles_of[m];
@@ -292,7 +313,8 @@ dd_status plan_batch(dd_batch *b, std::vector<TileDesc> (&tiles_of)[3]) {
       return fail(DD_ERR_CONFIGURATION, "batch LDS demand exceeds the device limit");
     int per_cu = 0;
     if (m == DD_STEP_TWO) {
-      if (b->compact) DD_HIP(rowsum_u16_batch_occupancy(k.g, &per_cu));
+      if (b->compact && k.g.pairs) DD_HIP(rowsum_u16_pairs_batch_occupancy(k.g, &per_cu));
+      else if (b->compact) DD_HIP(rowsum_u16_batch_occupancy(k.g, &per_cu));
       else DD_HIP(rowsum_f32_batch_occupancy(k.g, &per_cu));
     } else {
       DD_HIP(rowsum_u8_batch_occupancy(k.g, k.flush, &per_cu));
@@ -365,6 +387,19 @@ dd_status build(dd_batch *b) {
   return DD_OK;
 }
 
+// The compact step two's persistent and tile-list kernels: the paired form
+// when the geometry says so.
+hipError_t launch_u16_batch(const RowSumF32Params *d_jobs, const TileQueues &tq, const Geometry &g,
+                            uint32_t workgroups, hipStream_t s) {
+  return g.pairs ? launch_rowsum_u16_pairs_batch(d_jobs, tq, g, workgroups, s)
+                 : launch_rowsum_u16_batch(d_jobs, tq, g, workgroups, s);
+}
+hipError_t launch_u16_list(const RowSumF32Params *d_jobs, const TileDesc *d_tiles, uint32_t n, const Geometry &g,
+                           hipStream_t s) {
+  return g.pairs ? launch_rowsum_u16_pairs_list(d_jobs, d_tiles, n, g, s)
+                 : launch_rowsum_u16_list(d_jobs, d_tiles, n, g, s);
+}
+
 // The ordinary launches of kind m of the jobs that run beside the batch.
 dd_status enqueue_solo(dd_batch *b, int m, hipStream_t s) {
   dd_status st = DD_OK;
@@ -387,7 +422,7 @@ dd_status enqueue_kind(dd_batch *b, int m, hipStream_t s) {
   uint32_t *heads = b->heads.ptr + m * kQueues * kHeadStride;
   DD_HIP(launch_clear_words(heads, kQueues * kHeadStride, s));
   TileQueues tq{k.d_tiles.ptr, k.d_begin.ptr, heads};
-  if (m == DD_STEP_TWO && b->compact) DD_HIP(launch_rowsum_u16_batch(k.d_f32.ptr, tq, k.g, k.workgroups, s));
+  if (m == DD_STEP_TWO && b->compact) DD_HIP(launch_u16_batch(k.d_f32.ptr, tq, k.g, k.workgroups, s));
   else if (m == DD_STEP_TWO) DD_HIP(launch_rowsum_f32_batch(k.d_f32.ptr, tq, k.g, k.workgroups, s));
   else DD_HIP(launch_rowsum_u8_batch(k.d_u8.ptr, tq, k.g, k.flush, k.workgroups, s));
   return DD_OK;
@@ -417,7 +452,7 @@ dd_status enqueue_pipelined(dd_batch *b, hipStream_t s) {
     DD_HIP(hipEventRecord(b->events[3 + c], a));
     DD_HIP(hipStreamWaitEvent(c2, b->events[3 + c], 0));
     if (b->n2[c] && b->compact)
-      DD_HIP(launch_rowsum_u16_list(k2.d_f32.ptr, b->d_list2.ptr + b->off2[c], b->n2[c], g2, c2));
+      DD_HIP(launch_u16_list(k2.d_f32.ptr, b->d_list2.ptr + b->off2[c], b->n2[c], g2, c2));
     else if (b->n2[c])
       DD_HIP(launch_rowsum_f32_list(k2.d_f32.ptr, b->d_list2.ptr + b->off2[c], b->n2[c], g2, c2));
   }
@@ -521,7 +556,7 @@ dd_status batch_enqueue_static(dd_batch *b, uint8_t *dev, const StaticBatchImage
                   reinterpret_cast<const uint32_t *>(dev + img.begin[m]), heads};
     if (m == DD_STEP_TWO) {
       const auto *params = reinterpret_cast<const RowSumF32Params *>(dev + img.params[m]);
-      if (b->compact) DD_HIP(launch_rowsum_u16_batch(params, tq, k.g, k.workgroups, s));
+      if (b->compact) DD_HIP(launch_u16_batch(params, tq, k.g, k.workgroups, s));
       else DD_HIP(launch_rowsum_f32_batch(params, tq, k.g, k.workgroups, s));
     } else {
       const auto *params = reinterpret_cast<const RowSumU8Params *>(dev + img.params[m]);
@@ -650,6 +685,13 @@ dd_status dd_batch_set_compact_intermediate(dd_batch *batch, int enable) {
   return DD_OK;
 }
 
+dd_status dd_batch_set_paired_step_two(dd_batch *batch, int enable) {
+  if (!batch) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
+  batch->paired_requested = enable != 0;
+  batch->invalidate();
+  return DD_OK;
+}
+
 dd_status dd_batch_get_job_placement(dd_batch *batch, uint32_t job, uint32_t *beside, uint32_t *compact) {
   if (!batch || !beside || !compact) return fail(DD_ERR_INVALID_ARGUMENT, "null argument");
   if (job >= batch->jobs.size()) return fail(DD_ERR_INVALID_ARGUMENT, "job index out of range");
@@ -676,6 +718,7 @@ dd_status dd_batch_get_info(dd_batch *batch, dd_batch_info *info) {
   info->compact_intermediate = batch->compact ? 1 : 0;
   info->pipeline_chunks = (uint32_t)batch->n1.size();
   info->solo_jobs = (uint32_t)(batch->solo_single.size() + batch->solo_one.size());
+  info->paired_step_two = batch->compact && batch->q[DD_STEP_TWO].n_tiles > 0 && batch->q[DD_STEP_TWO].g.pairs;
   return DD_OK;
 }
 

@@ -125,7 +125,15 @@ struct RowSumF32Params {
   const uint32_t *rows_count;
   uint32_t rows_pitch;        // also the most rows of a slice
   uint64_t out_inner_pitch;   // floats between the inner indices of one outer index
+  // Paired uint16 step two (kernels_u16_pairs.hip): LDS windows per pair of
+  // subbands (1 .. kPairSlotsMax; at least the number of values the shift
+  // difference of the pair takes over any DM tile); `unroll` then counts
+  // pairs per barrier and `combine_rows` pairs added as 16-bit lanes.
+  uint32_t pair_slots;
 };
+
+// Most LDS windows per pair of subbands in the paired step two.
+constexpr uint32_t kPairSlotsMax = 4;
 
 // Geometry of one launch.
 struct Geometry {
@@ -138,6 +146,7 @@ struct Geometry {
   uint32_t staged;     // 1 = pipelined (double-buffered) kernel
   uint32_t packed;     // 1 = the rows may be packed (RowSumU8Params::in_log2f)
   uint32_t unroll;     // rows staged per barrier (the configuration's, or its cap)
+  uint32_t pairs;      // 1 = the paired uint16 step two (kernels_u16_pairs.hip)
 };
 
 // Launchers (kernels_u8.hip / kernels_f32.hip).  Return hipSuccess or the
@@ -198,6 +207,13 @@ hipError_t launch_rowsum_u16_staged(const RowSumF32Params &p, const Geometry &g,
 hipError_t launch_rowsum_u16_batch(const RowSumF32Params *d_jobs, const TileQueues &q, const Geometry &g,
                                    uint32_t workgroups, hipStream_t stream);
 hipError_t rowsum_u16_batch_occupancy(const Geometry &g, int *per_cu);
+// The same step two with subbands added in pairs first (kernels_u16_pairs.hip;
+// RowSumF32Params::pair_slots set by prepare_step_two_u16).
+hipError_t launch_rowsum_u16_pairs_batch(const RowSumF32Params *d_jobs, const TileQueues &q, const Geometry &g,
+                                         uint32_t workgroups, hipStream_t stream);
+hipError_t launch_rowsum_u16_pairs_list(const RowSumF32Params *d_jobs, const TileDesc *d_tiles, uint32_t n,
+                                        const Geometry &g, hipStream_t stream);
+hipError_t rowsum_u16_pairs_batch_occupancy(const Geometry &g, int *per_cu);
 inline uint32_t u16_staged_chunks(uint32_t run) { return run / 8 + 1; }  // 16-byte loads per lane per row
 constexpr uint32_t kStagedSlotsPerWave = 2;  // rows staged per wavefront per group
 constexpr uint32_t kU8StagedChunks = 2;      // 16-byte staging loads per lane per row (window <= 2 KiB)

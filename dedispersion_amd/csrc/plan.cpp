@@ -87,6 +87,7 @@ dd_status build_integer(dd_plan *p, ShiftTable &t, const float *d_table, uint32_
   t.rows = rows;
   t.dms = dms;
   t.spread_cache.clear();
+  t.pair_cache.clear();
   t.pitch = roundup(std::max(dms, 1u), 64) + 64;
   DD_HIP(t.dev.allocate((size_t)rows * t.pitch));
   if (dms == 0 || rows == 0) {
@@ -263,6 +264,7 @@ dd_status geometry_of(const dd_plan *p, dd_mode mode, const dd_conf &c, Geometry
   g.items_d0 = c.items_d0;
   g.items_d1 = c.items_d1;
   g.packed = 0;
+  g.pairs = 0;
   g.unroll = c.unroll;
   // rows of floats: step two, and the single step and step one over float input
   const bool float_rows = mode == DD_STEP_TWO || p->input_bits == 32;
@@ -558,7 +560,7 @@ dd_status prepare_step_two(dd_plan *plan, const dd_conf &c, const float *d_inter
 }
 
 dd_status prepare_step_two_u16(dd_plan *plan, const dd_conf &c, const void *d_intermediate, float *d_output,
-                               uint32_t first_sbeam, uint32_t nr_sbeams, F32Launch &L) {
+                               uint32_t first_sbeam, uint32_t nr_sbeams, F32Launch &L, bool pairs) {
   dd_status s = prepare_step_two(plan, c, reinterpret_cast<const float *>(d_intermediate), d_output, first_sbeam,
                                  nr_sbeams, L);
   if (s != DD_OK) return s;
@@ -584,6 +586,27 @@ dd_status prepare_step_two_u16(dd_plan *plan, const dd_conf &c, const void *d_in
   L.k.unroll = unroll;
   L.k.win_floats = win;
   L.k.combine_rows = 65535u / bound >= 4 ? 4 : 65535u / bound >= 2 ? 2 : 1;
+  L.k.pair_slots = 0;
+  L.g.pairs = 0;
+  if (!pairs || 2 * bound > 65535u) return DD_OK;
+  // the paired kernel: windows [2 buffers][pairs][slots][2 copies][win / 2
+  // dwords]; pairs per barrier at most the wavefronts (one staging wavefront
+  // or more per pair), half the configuration's rows, capped by LDS
+  const uint32_t slots = plan->ish_two.pair_range(tile);
+  const uint64_t aligned = L.k.in_row_pitch | L.k.in_coarse_pitch | L.k.in_beam_pitch;
+  if (slots > kPairSlotsMax || (aligned & 7) != 0) return DD_OK;
+  auto lds_pairs = [&](uint32_t gp) {
+    return 4 * (roundup(2 * plan->obs.nr_subbands, 4) + 2 * gp * slots * win + 2 * gp * tile);
+  };
+  uint32_t gp = std::max(1u, std::min(c.unroll / 2, c.threads_d1));
+  while (gp > 1 && lds_pairs(gp) > plan->max_lds) --gp;
+  if (lds_pairs(gp) > plan->max_lds) return DD_OK;
+  L.g.pairs = 1;
+  L.g.lds_bytes = lds_pairs(gp);
+  L.g.unroll = gp;
+  L.k.unroll = gp;
+  L.k.pair_slots = slots;
+  L.k.combine_rows = 65535u / (2 * bound) >= 2 ? 2 : 1;
   return DD_OK;
 }
 

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <string>
 #include <vector>
 
@@ -69,6 +70,27 @@ struct ShiftTable {
     return m;
   }
   mutable std::vector<std::pair<uint32_t, uint32_t>> spread_cache;
+  // Most values that the shift difference of a row pair (2k, 2k + 1) takes
+  // over the DMs of any tile of `tile` consecutive DMs, counted as the width
+  // of its range (cached per tile size like max_spread).
+  uint32_t pair_range(uint32_t tile) const {
+    for (auto &c : pair_cache)
+      if (c.first == tile) return c.second;
+    uint32_t m = 1;
+    for (uint32_t r = 0; r + 1 < rows; r += 2)
+      for (uint32_t d0 = 0; d0 < dms; d0 += tile) {
+        int64_t lo = INT64_MAX, hi = INT64_MIN;
+        for (uint32_t d = d0; d < std::min(d0 + tile, dms); ++d) {
+          const int64_t delta = (int64_t)at(r + 1, d) - (int64_t)at(r, d);
+          lo = std::min(lo, delta);
+          hi = std::max(hi, delta);
+        }
+        m = (uint32_t)std::max<int64_t>(m, hi - lo + 1);
+      }
+    pair_cache.emplace_back(tile, m);
+    return m;
+  }
+  mutable std::vector<std::pair<uint32_t, uint32_t>> pair_cache;
 };
 
 } // namespace ddamd
@@ -140,9 +162,13 @@ dd_status prepare_step_two(dd_plan *plan, const dd_conf &c, const float *d_inter
 // Step two over the uint16 sums that step one writes with out_u16 (a batch
 // that owns its intermediate): the launch of prepare_step_two with `in` read
 // as uint16 elements.  Fails with DD_ERR_CONFIGURATION when c does not select
-// the staged pipeline or a subband's sum could reach 2^16.
+// the staged pipeline or a subband's sum could reach 2^16.  With `pairs`, the
+// paired kernel (kernels_u16_pairs.hip) is prepared when the job allows it
+// (a pair's sum below 2^16, at most kPairSlotsMax shift differences per pair
+// and DM tile, rows 8-element aligned, the windows within LDS); L.g.pairs
+// says which was prepared.
 dd_status prepare_step_two_u16(dd_plan *plan, const dd_conf &c, const void *d_intermediate, float *d_output,
-                               uint32_t first_sbeam, uint32_t nr_sbeams, F32Launch &L);
+                               uint32_t first_sbeam, uint32_t nr_sbeams, F32Launch &L, bool pairs = false);
 // The single step and step one over float input rows (a plan with
 // input_bits 32): the float row-sum kernels over the live-channel lists.
 dd_status prepare_single_f32(dd_plan *plan, const dd_conf &c, const float *d_input, float *d_output,

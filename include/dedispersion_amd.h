@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define DD_ABI_VERSION 4
+#define DD_ABI_VERSION 5
 
 typedef enum dd_status {
   DD_OK = 0,
@@ -322,6 +322,7 @@ typedef struct dd_batch_info {
   uint32_t compact_intermediate; /* 1: step one writes uint16 sums (see dd_batch_set_compact_intermediate) */
   uint32_t pipeline_chunks;      /* chunks of the pipelined mode in effect, 0 = off */
   uint32_t solo_jobs;            /* jobs run beside the batch kernels (see dd_batch_set_conf) */
+  uint32_t paired_step_two;      /* 1: the compact step two adds subbands in pairs (see dd_batch_set_paired_step_two) */
 } dd_batch_info;
 
 dd_status dd_batch_create(dd_batch **batch);
@@ -352,6 +353,18 @@ dd_status dd_batch_set_conf(dd_batch *batch, dd_mode mode, const dd_conf *conf);
  * buffers then hold uint16 sums.  With enable = 0, or when a job does not
  * qualify, the batch keeps the float intermediate of the reference layout. */
 dd_status dd_batch_set_compact_intermediate(dd_batch *batch, int enable);
+/* Paired step two (off by default), with the compact intermediate: step two
+ * first adds the uint16 sums of subbands 2k and 2k + 1 for every shift
+ * difference the pair takes over a tile's DMs, and each DM then adds one
+ * row per pair.  It applies when, for every job in the batch kernels, twice
+ * 255 x the live channels of a subband is below 2^16, a pair's shift
+ * difference takes at most 4 values per DM tile and the intermediate rows are
+ * 8-element aligned; otherwise every subband is added on its own.  Results
+ * are bit-identical either way.  On the 40-beam MI355X workload it is
+ * slower than adding every subband (2.68 against 2.22 ms for the batch's step
+ * two), so it is an option.  No counterpart in the reference, whose step two
+ * (include/Dedispersion.hpp:155-176) adds every subband. */
+dd_status dd_batch_set_paired_step_two(dd_batch *batch, int enable);
 /* Pipelined mode (chunks >= 2, the default is 3; 0 or 1 = off): the two-step jobs are split,
  * in order, into `chunks` groups of near-equal work.  Step one of group c
  * and step two of group c run on two batch-owned streams, ordered by an

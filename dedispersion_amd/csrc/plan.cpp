@@ -170,7 +170,8 @@ dd_conf default_conf(dd_mode mode) {
     c.threads_d1 = 8;
     c.items_d0 = 8;
     c.items_d1 = 4;
-    c.unroll = 16;
+    c.unroll = 8;
+    c.local = 2;
     break;
   }
   return c;

@@ -239,8 +239,9 @@ def tuned_lines_from_records(records: List[Dict], device: str = "MI355X") -> Dic
 TUNED_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
 # Exhaustive sweeps of the plan configurations (every kernel of the library)
 # over the 128-channel, ARTS and 40-beam layouts: the source of the
-# reference-format tuned files.
-PLAN_RECORDS_PATH = os.path.join(TUNED_DIR, "sweep_plans_mi355x.jsonl.gz")
+# reference-format tuned files and the learned tuner's training and
+# evaluation records.
+PLAN_RECORDS_PATH = os.path.join(TUNED_DIR, "sweep_mi355x.jsonl.gz")
 
 
 def tuned_file_path(mode: str, device: str = "MI355X") -> str:

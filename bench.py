@@ -7,8 +7,11 @@ element of the final dedispersed output (beam x DM trial x sample).
 Default workload: 40 synthesized beams of 1536 channels with seeded uneven
 DM counts {512..4096} and sample counts {8192..32768}, two-step subband
 dedispersion (32 subbands, coarse DMs of 6.4, 32 fine DMs of 0.2), run as a
-batch replayed from a HIP graph: the beams are split into chunks, step one of
-a chunk and step two of the previous chunk run concurrently on two streams.
+batch replayed from a HIP graph: one persistent launch of step one over the
+tiles of all beams, then one of step two (--chunks N >= 2 runs the pipelined
+form instead: the beams split into N chunks, step one of a chunk beside step
+two of the previous one on two streams).  The per-kernel durations are those
+of the two persistent kernels of the timed graph, launched the same way.
 The same work with one launch per beam and step, each beam with its plan's
 tuned configuration and the reference's float intermediate (the reference's
 launch pattern), is timed beside it.
@@ -48,8 +51,9 @@ def parse():
                     choices=["batched-beams", "arts-two-step", "arts-single", "detect-peaks"])
     ap.add_argument("--input-bits", type=int, default=8, choices=[1, 2, 4, 8],
                     help="bits per input sample (1, 2, 4 = packed rows of uniform random fields)")
-    ap.add_argument("--chunks", type=int, default=None,
-                    help="batch pipeline depth (default: the packaged tuning's; 0 = persistent launches in order)")
+    ap.add_argument("--chunks", type=int, default=0,
+                    help="batch pipeline depth (0 = one persistent launch per kernel kind, in order; "
+                         ">= 2 = the pipelined form; -1 = the packaged tuning's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -284,7 +288,7 @@ def run_batched(args, torch):
     from dedispersion_amd.beams import BeamSet
 
     ws = workloads.batched_beams()
-    bs = BeamSet(torch, ws, input_bits=args.input_bits, chunks=args.chunks)
+    bs = BeamSet(torch, ws, input_bits=args.input_bits, chunks=None if args.chunks < 0 else args.chunks)
     info = bs.info()
     units = info["units"]
     log(f"batched beams: {info}")

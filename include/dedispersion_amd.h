@@ -261,9 +261,9 @@ dd_status dd_subband_step_two(dd_plan *plan, const float *d_intermediate, float 
  *
  * No counterpart in the reference, whose output is the dedispersed array;
  * this is the search a pipeline runs next.  For each of nr_rows rows of
- * nr_samples integer-valued floats (the outputs above; a fractional part is
- * truncated; values below 2^24, and nr_samples x max^2 below 2^63), pitch
- * floats apart, it finds the boxcar of width w in {1, 2, 4, ...,
+ * nr_samples non-negative integer-valued floats (the outputs above; a
+ * fractional part is truncated; values below 2^24, and nr_samples x max^2
+ * below 2^63), pitch floats apart, it finds the boxcar of width w in {1, 2, 4, ...,
  * 2^(nr_widths - 1)} (nr_widths <= 7) and start t (t + w <= nr_samples)
  * with the highest
  *     snr = (sum_{i<w} y(t + i) - w * mean) / (sigma * sqrt(w)),

@@ -46,6 +46,66 @@ def test_random_rows_against_oracle(torch_cuda, S, pitch, nw):
     compare(got, want, f"S={S} widths={nw}")
 
 
+# The search walks each row in chunks of 63 runs of 68 starts (4284 starts),
+# reading 4352 samples: sizes around those borders, pulses straddling runs
+# and chunks, equal maxima in different runs and chunks (the first must win),
+# maxima at the row's last window, zero and constant rows.
+@pytest.mark.parametrize("S", [1, 67, 68, 131, 4284, 4285, 4352, 8568, 8569, 12859])
+@pytest.mark.parametrize("aligned", [True, False], ids=["aligned", "unaligned"])
+def test_chunk_and_run_borders(torch_cuda, S, aligned):
+    pitch = (S + 3) // 4 * 4 if aligned else S + (1 if S % 4 == 0 else 0)
+    rng = np.random.default_rng(S * 2 + aligned)
+    rows = rng.integers(0, 200, size=(40, pitch)).astype(np.float32)
+    borders = [b for b in (67, 68, 135, 136, 4283, 4284, 4351, 4352, 8567, 8568, 12851, 12852) if b < S]
+    for r in range(24):
+        w = 1 << (r % 7)
+        if borders:
+            t0 = max(0, min(S - w, borders[r % len(borders)] - w // 2))
+            rows[r, t0:t0 + w] += 5000
+    # equal maxima: an impulse in a late run and the same in a later chunk
+    if S > 10:
+        rows[24, :] = 3
+        rows[24, S // 3] = rows[24, S - 2] = 900
+    # the row's last window is the best of its width
+    rows[25, :] = 1
+    rows[25, max(0, S - 64):S] = 400
+    rows[26, :] = 0                       # zero row
+    rows[27, :] = 9                       # constant row
+    rows[28, :] = 0
+    rows[28, S - 1] = 50                  # a spike in the last sample only
+    got = run(torch_cuda, rows, S, 7)
+    want = oracle.detect_peaks(rows, S, 7)
+    compare(got, want, f"S={S} pitch={pitch}")
+
+
+def test_rows_beyond_one_pass_of_the_grid(torch_cuda):
+    """More rows than the launch has wavefronts (each takes several rows)."""
+    rng = np.random.default_rng(11)
+    S, pitch, n = 300, 304, 2 * 256 * 8 + 37
+    rows = rng.integers(0, 1000, size=(n, pitch)).astype(np.float32)
+    for r in range(0, n, 7):
+        w = 1 << (r % 7)
+        t0 = (r * 13) % (S - w)
+        rows[r, t0:t0 + w] += 3000
+    got = run(torch_cuda, rows, S, 7)
+    want = oracle.detect_peaks(rows, S, 7)
+    compare(got, want, f"{n} rows")
+
+
+def test_long_rows_against_oracle(torch_cuda):
+    """Rows of the 40-beam outputs' longest length, values of 1536-channel sums."""
+    rng = np.random.default_rng(12)
+    S = 32768
+    rows = rng.integers(180000, 210000, size=(48, S)).astype(np.float32)
+    for r in range(48):
+        w = 1 << (r % 7)
+        t0 = int(rng.integers(0, S - w))
+        rows[r, t0:t0 + w] += 20000
+    got = run(torch_cuda, rows, S, 7)
+    want = oracle.detect_peaks(rows, S, 7)
+    compare(got, want, "32768-sample rows")
+
+
 @pytest.mark.parametrize("dm_index,t0,width", [(40, 1000, 1), (17, 2500, 6), (55, 3000, 16)])
 def test_pulse_found_in_dedispersed_output(torch_cuda, dm_index, t0, width):
     """Single step, 256 channels, 64 DMs of 0.5, 4096 samples of Gaussian

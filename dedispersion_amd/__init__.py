@@ -31,6 +31,7 @@ __all__ = [
     "synth_gaussian_u8",
     "synth_uniform_u8",
     "detect_peaks",
+    "detect_peaks_batch",
     "PEAK_DTYPE",
 ]
 
@@ -363,3 +364,15 @@ def detect_peaks(d_rows: int, nr_rows: int, nr_samples: int, pitch: int, nr_widt
     nr_samples integer-valued floats, pitch floats apart; d_peaks receives
     nr_rows dd_peak records (PEAK_DTYPE)."""
     check(load().dd_detect_peaks(d_rows, nr_rows, nr_samples, pitch, nr_widths, d_peaks, stream or None))
+
+
+def detect_peaks_batch(jobs, nr_widths: int, stream: int = 0) -> None:
+    """The same search over many sets of rows in one launch per 64 sets
+    (dd_detect_peaks_batch).  jobs: (d_rows, nr_rows, nr_samples, pitch,
+    d_peaks) per set."""
+    from ._lib import DDDetectJob
+
+    arr = (DDDetectJob * max(1, len(jobs)))()
+    for a, (d_rows, nr_rows, nr_samples, pitch, d_peaks) in zip(arr, jobs):
+        a.d_rows, a.nr_rows, a.nr_samples, a.pitch, a.d_peaks = d_rows, nr_rows, nr_samples, pitch, d_peaks
+    check(load().dd_detect_peaks_batch(arr, len(jobs), nr_widths, stream or None))

@@ -100,6 +100,17 @@ class DDPeak(ctypes.Structure):
     ]
 
 
+class DDDetectJob(ctypes.Structure):
+    _fields_ = [
+        ("d_rows", c_void_p),
+        ("nr_rows", c_uint64),
+        ("pitch", c_uint64),
+        ("nr_samples", c_uint32),
+        ("reserved", c_uint32),
+        ("d_peaks", c_void_p),
+    ]
+
+
 class DDBatchInfo(ctypes.Structure):
     _fields_ = [
         ("jobs", c_uint32),
@@ -165,6 +176,7 @@ PROTOTYPES = {
     "dd_subband_step_one_f32": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "dd_subband_step_two": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_void_p]),
     "dd_detect_peaks": (c_int, [c_void_p, c_uint64, c_uint32, c_uint64, c_uint32, c_void_p, c_void_p]),
+    "dd_detect_peaks_batch": (c_int, [POINTER(DDDetectJob), c_uint32, c_uint32, c_void_p]),
     "dd_tune_time": (
         c_int,
         [c_void_p, c_int, POINTER(DDConf), c_void_p, c_void_p, c_uint32, c_uint32, c_void_p, POINTER(c_float),

@@ -65,25 +65,35 @@ constexpr uint32_t kRescan = kRun + kHalo;    // samples of one run's windows (1
 
 static_assert(kRun % 4 == 0 && kRun >= kHalo, "a run covers the reach of the widest window");
 
-struct DetectParams {
+// One set of rows: nr_rows rows of n_samples floats, pitch floats apart.
+struct DetectJob {
   const float *rows;
-  uint64_t pitch;       // floats between rows
-  uint64_t n_rows;
-  uint32_t n_samples;
-  uint32_t n_widths;    // widths 1 .. 2^(n_widths - 1)
-  uint32_t chunks;      // chunks per row, ceil(n_samples / kStep)
+  uint64_t pitch;
   dd_peak *out;
+  uint32_t n_samples;
+  uint32_t chunks;      // chunks per row, ceil(n_samples / kStep)
+};
+
+constexpr uint32_t kMaxJobs = 64;  // sets of rows per launch (kernel arguments)
+
+// The rows of all jobs in a launch, numbered job after job: job j holds rows
+// first_row[j] .. first_row[j + 1] - 1.
+struct DetectParams {
+  DetectJob jobs[kMaxJobs];
+  uint64_t first_row[kMaxJobs + 1];
+  uint32_t n_jobs;
+  uint32_t n_widths;    // widths 1 .. 2^(n_widths - 1)
 };
 
 // Reads the kStage samples of chunk c of a row, piece j of lane l being
 // samples 4 (64 j + l) .. +3 of the chunk: 16-byte loads when the rows are
 // 16-byte aligned (VEC), else single floats.  Zeros past the row's end.
 template <bool VEC>
-__device__ __forceinline__ void load_chunk(const DetectParams &p, uint64_t row, uint32_t c, uint32_t lane,
+__device__ __forceinline__ void load_chunk(const DetectJob &job, uint64_t row, uint32_t c, uint32_t lane,
                                            u32x4 (&v)[kVec]) {
   const uint32_t s0 = c * kStep;
-  const uint32_t left = min(p.n_samples - s0, kStage);  // samples of the row in this chunk
-  const float *base = p.rows + row * p.pitch + s0;
+  const uint32_t left = min(job.n_samples - s0, kStage);  // samples of the row in this chunk
+  const float *base = job.rows + row * job.pitch + s0;
   const auto exact = make_rsrc(base, 4 * left);
   if (VEC) {
     // whole pieces inside the row; the piece holding the row's last samples,
@@ -272,19 +282,23 @@ __device__ uint32_t first_start(const float *y, uint32_t S, uint32_t P, uint32_t
   return P;  // not reached: M occurs in this run
 }
 
+// One wavefront per workgroup.  Wavefront b takes rows b, b + gridDim.x, ...
+// of the launch's numbering, each whole, chunk after chunk; the next chunk
+// (of this row or the next) is loaded while this one is searched.
 template <bool VEC> __global__ void __launch_bounds__(64) detect_peaks(const DetectParams p) {
   __shared__ __attribute__((aligned(16))) uint32_t stage[kStage];
   const uint32_t lane = threadIdx.x;
-  const uint32_t S = p.n_samples, C = p.chunks;
-  const uint64_t G = gridDim.x;
-  uint64_t row = blockIdx.x;
-  uint32_t c = 0;
-  if (row >= p.n_rows) return;
+  const uint64_t G = gridDim.x, n_rows = p.first_row[p.n_jobs];
+  uint64_t g = blockIdx.x;  // row of the launch
+  if (g >= n_rows) return;
+  uint32_t j = 0;           // its job
+  while (g >= p.first_row[j + 1]) ++j;
+  uint32_t c = 0;           // its chunk
 
   // aligned rows: the next chunk's loads are in flight while this one is
   // searched; single-float reads (unaligned rows) load each chunk in turn
   u32x4 pre[kVec];
-  if (VEC) load_chunk<VEC>(p, row, 0, lane, pre);
+  if (VEC) load_chunk<VEC>(p.jobs[j], g - p.first_row[j], 0, lane, pre);
 
   uint32_t best[kWidths], pos[kWidths];
   uint64_t sum = 0, sq = 0;
@@ -295,28 +309,32 @@ template <bool VEC> __global__ void __launch_bounds__(64) detect_peaks(const Det
   }
 
   for (;;) {
-    if (!VEC) load_chunk<VEC>(p, row, c, lane, pre);
+    const DetectJob &job = p.jobs[j];
+    const uint64_t row = g - p.first_row[j];
+    if (!VEC) load_chunk<VEC>(job, row, c, lane, pre);
     // ---- stage this chunk as integers, then issue the next chunk's loads ----
 #pragma unroll
-    for (int j = 0; j < kVec; ++j) {
+    for (int k = 0; k < kVec; ++k) {
       u32x4 v;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         // through a scalar: __builtin_bit_cast of a vector element reads
         // element 0 whatever e is (hipcc 7.2)
-        const uint32_t bits = pre[j][e];
+        const uint32_t bits = pre[k][e];
         v[e] = (uint32_t)__builtin_bit_cast(float, bits);
       }
-      *reinterpret_cast<u32x4 *>(stage + 4 * (64 * j + lane)) = v;
+      *reinterpret_cast<u32x4 *>(stage + 4 * (64 * k + lane)) = v;
     }
-    uint64_t nrow = row;
-    uint32_t nc = c + 1;
-    if (nc == C) {
-      nrow += G;
+    const bool row_end = c + 1 == job.chunks;
+    uint64_t ng = g;
+    uint32_t nj = j, nc = c + 1;
+    if (row_end) {
+      ng += G;
       nc = 0;
+      while (nj < p.n_jobs && ng >= p.first_row[nj + 1]) ++nj;
     }
-    const bool more = nrow < p.n_rows;
-    if (VEC && more) load_chunk<VEC>(p, nrow, nc, lane, pre);
+    const bool more = ng < n_rows;
+    if (VEC && more) load_chunk<VEC>(p.jobs[nj], ng - p.first_row[nj], nc, lane, pre);
     __syncthreads();
 
     // ---- search this lane's run ----
@@ -337,7 +355,8 @@ template <bool VEC> __global__ void __launch_bounds__(64) detect_peaks(const Det
     __syncthreads();
 
     // ---- the row's end: reduce, pick the width, find its first start ----
-    if (c == C - 1) {
+    if (row_end) {
+      const uint32_t S = job.n_samples;
       const uint64_t tsum = wave_sum64(sum), tsq = wave_sum64(sq);
       uint32_t M[kWidths], P[kWidths];
 #pragma unroll
@@ -352,7 +371,7 @@ template <bool VEC> __global__ void __launch_bounds__(64) detect_peaks(const Det
         Pk = k.w == (uint32_t)w ? P[w] : Pk;
         Mk = k.w == (uint32_t)w ? M[w] : Mk;
       }
-      const uint32_t t = first_start(p.rows + row * p.pitch, S, Pk, 1u << k.w, Mk, lane);
+      const uint32_t t = first_start(job.rows + row * job.pitch, S, Pk, 1u << k.w, Mk, lane);
       if (lane == 0) {
         dd_peak o;
         o.snr = (float)k.snr;
@@ -360,7 +379,7 @@ template <bool VEC> __global__ void __launch_bounds__(64) detect_peaks(const Det
         o.sigma = (float)k.sigma;
         o.sample = t;
         o.width = 1u << k.w;
-        p.out[row] = o;
+        job.out[row] = o;
       }
       sum = sq = 0;
 #pragma unroll
@@ -370,7 +389,8 @@ template <bool VEC> __global__ void __launch_bounds__(64) detect_peaks(const Det
       }
     }
     if (!more) break;
-    row = nrow;
+    g = ng;
+    j = nj;
     c = nc;
   }
 }
@@ -380,37 +400,99 @@ dd_status fail_detect(dd_status s, const std::string &msg) {
   return s;
 }
 
+dd_status check_job(const dd_detect_job &j) {
+  if (j.nr_rows == 0) return DD_OK;
+  if (!j.d_rows || !j.d_peaks) return fail_detect(DD_ERR_INVALID_ARGUMENT, "null argument");
+  if (j.nr_samples == 0 || j.pitch < j.nr_samples)
+    return fail_detect(DD_ERR_INVALID_ARGUMENT, "need 0 < nr_samples <= pitch");
+  if (j.nr_samples > 0xFFFFFFFFu - kStage)
+    return fail_detect(DD_ERR_INVALID_ARGUMENT, "nr_samples too large for the search's 32-bit starts");
+  return DD_OK;
+}
+
+// 16-byte loads need the rows 16-byte aligned and a pitch of whole float4s.
+bool aligned(const dd_detect_job &j) {
+  return j.pitch % 4 == 0 && (reinterpret_cast<uintptr_t>(j.d_rows) & 15) == 0;
+}
+
+// One launch over up to kMaxJobs jobs (all aligned, or all not).
+dd_status launch(const dd_detect_job *const *jobs, uint32_t n, uint32_t nr_widths, bool vec, hipStream_t stream) {
+  DetectParams p{};
+  p.n_jobs = n;
+  p.n_widths = nr_widths;
+  for (uint32_t i = 0; i < n; ++i) {
+    const dd_detect_job &j = *jobs[i];
+    p.jobs[i].rows = j.d_rows;
+    p.jobs[i].pitch = j.pitch;
+    p.jobs[i].out = j.d_peaks;
+    p.jobs[i].n_samples = j.nr_samples;
+    p.jobs[i].chunks = (j.nr_samples + kStep - 1) / kStep;
+    p.first_row[i + 1] = p.first_row[i] + j.nr_rows;
+  }
+  const uint64_t rows = p.first_row[n];
+  if (rows == 0) return DD_OK;
+  int device = 0, cus = 256;
+  (void)hipGetDevice(&device);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  // eight wavefronts per CU are resident (two per SIMD, by registers)
+  const uint64_t grid = std::min<uint64_t>(rows, (uint64_t)cus * 8);
+  auto kernel = vec ? detect_peaks<true> : detect_peaks<false>;
+  hipLaunchKernelGGL(kernel, dim3((uint32_t)grid), dim3(64), 0, stream, p);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail_detect(DD_ERR_HIP, std::string("detect_peaks launch: ") + hipGetErrorString(e));
+  return DD_OK;
+}
+
 } // namespace
 
 extern "C" {
+
+dd_status dd_detect_peaks_batch(const dd_detect_job *jobs, uint32_t nr_jobs, uint32_t nr_widths, void *stream) {
+  if (nr_jobs == 0) return DD_OK;
+  if (!jobs) return fail_detect(DD_ERR_INVALID_ARGUMENT, "null argument");
+  if (nr_widths < 1 || nr_widths > (uint32_t)kWidths)
+    return fail_detect(DD_ERR_INVALID_ARGUMENT, "nr_widths must be in [1, 7] (widths 1 .. 64)");
+  for (uint32_t i = 0; i < nr_jobs; ++i) {
+    const dd_status s = check_job(jobs[i]);
+    if (s != DD_OK) return fail_detect(s, "job " + std::to_string(i) + ": " + dd_last_error());
+  }
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  for (int vec = 1; vec >= 0; --vec) {
+    const dd_detect_job *group[kMaxJobs];
+    uint32_t n = 0;
+    for (uint32_t i = 0; i < nr_jobs; ++i) {
+      if (jobs[i].nr_rows == 0 || aligned(jobs[i]) != (vec == 1)) continue;
+      group[n++] = &jobs[i];
+      if (n == kMaxJobs) {
+        const dd_status s = launch(group, n, nr_widths, vec == 1, st);
+        if (s != DD_OK) return s;
+        n = 0;
+      }
+    }
+    if (n) {
+      const dd_status s = launch(group, n, nr_widths, vec == 1, st);
+      if (s != DD_OK) return s;
+    }
+  }
+  return DD_OK;
+}
 
 dd_status dd_detect_peaks(const float *d_rows, uint64_t nr_rows, uint32_t nr_samples, uint64_t pitch,
                           uint32_t nr_widths, dd_peak *d_peaks, void *stream) {
   if (!d_rows || !d_peaks) return fail_detect(DD_ERR_INVALID_ARGUMENT, "null argument");
   if (nr_rows == 0) return DD_OK;
-  if (nr_samples == 0 || pitch < nr_samples) return fail_detect(DD_ERR_INVALID_ARGUMENT, "need 0 < nr_samples <= pitch");
   if (nr_widths < 1 || nr_widths > (uint32_t)kWidths)
     return fail_detect(DD_ERR_INVALID_ARGUMENT, "nr_widths must be in [1, 7] (widths 1 .. 64)");
-  if (nr_samples > 0xFFFFFFFFu - kStage)
-    return fail_detect(DD_ERR_INVALID_ARGUMENT, "nr_samples too large for the search's 32-bit starts");
-  DetectParams p{};
-  p.rows = d_rows;
-  p.pitch = pitch;
-  p.n_rows = nr_rows;
-  p.n_samples = nr_samples;
-  p.n_widths = nr_widths;
-  p.chunks = (nr_samples + kStep - 1) / kStep;
-  p.out = d_peaks;
-  const bool vec = (pitch % 4 == 0) && ((reinterpret_cast<uintptr_t>(d_rows) & 15) == 0);
-  int device = 0, cus = 256;
-  (void)hipGetDevice(&device);
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
-  const uint64_t grid = std::min<uint64_t>(nr_rows, (uint64_t)cus * 8);
-  auto kernel = vec ? detect_peaks<true> : detect_peaks<false>;
-  hipLaunchKernelGGL(kernel, dim3((uint32_t)grid), dim3(64), 0, static_cast<hipStream_t>(stream), p);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail_detect(DD_ERR_HIP, std::string("detect_peaks launch: ") + hipGetErrorString(e));
-  return DD_OK;
+  dd_detect_job j{};
+  j.d_rows = d_rows;
+  j.nr_rows = nr_rows;
+  j.pitch = pitch;
+  j.nr_samples = nr_samples;
+  j.d_peaks = d_peaks;
+  const dd_status s = check_job(j);
+  if (s != DD_OK) return s;
+  const dd_detect_job *one = &j;
+  return launch(&one, 1, nr_widths, aligned(j), static_cast<hipStream_t>(stream));
 }
 
 } // extern "C"

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define DD_ABI_VERSION 5
+#define DD_ABI_VERSION 6
 
 typedef enum dd_status {
   DD_OK = 0,
@@ -281,6 +281,22 @@ typedef struct dd_peak {
 
 dd_status dd_detect_peaks(const float *d_rows, uint64_t nr_rows, uint32_t nr_samples, uint64_t pitch,
                           uint32_t nr_widths, dd_peak *d_peaks, void *stream);
+
+/* The same search over many sets of rows (the outputs of many beams, sizes
+ * free to differ) in one persistent launch per 64 sets: the same results as
+ * one dd_detect_peaks call per set, without a launch's ramp and drain per
+ * set.  A set with nr_rows 0 is skipped.  Also no counterpart in the
+ * reference.  `jobs` is host memory, read before the call returns. */
+typedef struct dd_detect_job {
+  const float *d_rows;
+  uint64_t nr_rows;
+  uint64_t pitch;       /* floats between rows */
+  uint32_t nr_samples;
+  uint32_t reserved;    /* 0 */
+  dd_peak *d_peaks;     /* nr_rows entries */
+} dd_detect_job;
+
+dd_status dd_detect_peaks_batch(const dd_detect_job *jobs, uint32_t nr_jobs, uint32_t nr_widths, void *stream);
 
 /* ---- tuning harness (replaces src/DedispersionTuning.cpp:301-428) ---- */
 

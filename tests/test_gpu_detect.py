@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import oracle
-from dedispersion_amd import PEAK_DTYPE, Plan, detect_peaks, workloads
+from dedispersion_amd import PEAK_DTYPE, Plan, detect_peaks, detect_peaks_batch, workloads
 from tests.conftest import host_threads
 from tests.gpu_helpers import stream_of
 
@@ -92,6 +92,22 @@ def test_rows_beyond_one_pass_of_the_grid(torch_cuda):
     compare(got, want, f"{n} rows")
 
 
+def test_rows_shared_by_wavefronts_over_several_rows(torch_cuda):
+    """Few enough rows that each is split over three wavefronts (three chunks
+    of 4284 starts), and more rows than workgroups, so a workgroup merges
+    several rows in turn."""
+    rng = np.random.default_rng(13)
+    S, n = 9000, 700
+    rows = rng.integers(0, 3000, size=(n, S)).astype(np.float32)
+    for r in range(n):
+        w = 1 << (r % 7)
+        t0 = (r * 4283 + 17) % (S - w)
+        rows[r, t0:t0 + w] += 9000
+    got = run(torch_cuda, rows, S, 7)
+    want = oracle.detect_peaks(rows, S, 7)
+    compare(got, want, f"{n} rows of {S}")
+
+
 def test_long_rows_against_oracle(torch_cuda):
     """Rows of the 40-beam outputs' longest length, values of 1536-channel sums."""
     rng = np.random.default_rng(12)
@@ -138,6 +154,72 @@ def test_pulse_found_in_dedispersed_output(torch_cuda, dm_index, t0, width):
     assert start < t0 + width and t0 < start + found, pk
     assert np.argmax(peaks["snr"]) in near, peaks["snr"][near]
     plan.close()
+
+
+def _batch_case(torch, shapes, seed):
+    """Device arrays of random rows with pulses, one per (rows, S, pitch)."""
+    rng = np.random.default_rng(seed)
+    host, dev, peaks = [], [], []
+    for n, S, pitch in shapes:
+        rows = rng.integers(0, 5000, size=(max(n, 1), pitch)).astype(np.float32)
+        for r in range(n):
+            w = 1 << (r % 7)
+            if S > w:
+                t0 = int(rng.integers(0, S - w))
+                rows[r, t0:t0 + w] += 20000
+        host.append(rows)
+        dev.append(torch.from_numpy(rows.ravel()).cuda())
+        peaks.append(torch.zeros(max(n, 1) * PEAK_DTYPE.itemsize, dtype=torch.uint8, device="cuda"))
+    jobs = [(d.data_ptr(), n, S, pitch, pk.data_ptr()) for d, pk, (n, S, pitch) in zip(dev, peaks, shapes)]
+    return host, dev, peaks, jobs
+
+
+def test_batch_matches_single_sets_and_oracle(torch_cuda):
+    """One batch over sets of different lengths, pitches (aligned and not)
+    and row counts (one empty): each set's peaks equal one dd_detect_peaks
+    call's, record for record, and the oracle's."""
+    torch = torch_cuda
+    shapes = [(40, 8192, 8192), (7, 30000, 30001), (0, 100, 100), (300, 4285, 4288), (3, 1, 1), (25, 12859, 12860),
+              (64, 700, 703)]
+    host, dev, peaks, jobs = _batch_case(torch, shapes, 21)
+    detect_peaks_batch(jobs, 7, stream_of(torch))
+    torch.cuda.synchronize()
+    for (ptr, n, S, pitch, pk), rows, d in zip(jobs, host, peaks):
+        if n == 0:
+            assert not d.any().item()
+            continue
+        got = d.cpu().numpy().view(PEAK_DTYPE)
+        single = torch.zeros_like(d)
+        detect_peaks(ptr, n, S, pitch, 7, single.data_ptr(), stream_of(torch))
+        torch.cuda.synchronize()
+        assert np.array_equal(got.view(np.uint8), single.cpu().numpy()), f"S={S}: batch differs from a single call"
+        compare(got, oracle.detect_peaks(rows[:n], S, 7), f"batch set S={S} pitch={pitch}")
+
+
+def test_batch_of_more_sets_than_one_launch_holds(torch_cuda):
+    """150 sets (three launches of at most 64), widths 1-16."""
+    torch = torch_cuda
+    rng = np.random.default_rng(22)
+    shapes = [(int(rng.integers(1, 6)), int(S), int(S) + int(rng.integers(0, 4)))
+              for S in rng.integers(50, 9000, size=150)]
+    host, dev, peaks, jobs = _batch_case(torch, shapes, 23)
+    detect_peaks_batch(jobs, 5, stream_of(torch))
+    torch.cuda.synchronize()
+    for (ptr, n, S, pitch, pk), rows, d in zip(jobs, host, peaks):
+        compare(d.cpu().numpy().view(PEAK_DTYPE), oracle.detect_peaks(rows[:n], S, 5), f"S={S} pitch={pitch}")
+
+
+def test_batch_rejects_bad_sets(torch_cuda):
+    from dedispersion_amd import DDError
+
+    x = torch_cuda.zeros(256, dtype=torch_cuda.float32, device="cuda")
+    good = (x.data_ptr(), 1, 64, 64, x.data_ptr())
+    with pytest.raises(DDError, match="nr_widths"):
+        detect_peaks_batch([good], 8)
+    with pytest.raises(DDError, match="job 1: need 0 < nr_samples <= pitch"):
+        detect_peaks_batch([good, (x.data_ptr(), 1, 65, 64, x.data_ptr())], 3)
+    with pytest.raises(DDError, match="job 0: null argument"):
+        detect_peaks_batch([(0, 1, 64, 64, x.data_ptr())], 3)
 
 
 def test_rejects_bad_arguments(torch_cuda):

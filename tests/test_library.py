@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
     lib = _lib.load()
     for name in declared_functions():
         assert hasattr(lib, name), name
-    assert lib.dd_abi_version() == 5
+    assert lib.dd_abi_version() == 6
     assert lib.dd_status_string(5) == b"invalid kernel configuration"
 
 

@@ -58,7 +58,38 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run --steps plain device-to-device copies of BYTES (for scaling HBM counters)")
+    ap.add_argument("--read-calibration", type=int, default=0, metavar="BYTES",
+                    help="only time --steps plain streaming reads of BYTES (dd_read_probe): the read bandwidth a "
+                         "streaming kernel reaches on this device")
     return ap.parse_args()
+
+
+def read_calibration(args, torch):
+    """The read bandwidth of a plain streaming read of a given size
+    (dd_read_probe: coalesced 16-byte loads, one partial sum per thread), the
+    practical ceiling of a read-bound kernel beside the 8 TB/s peak."""
+    from dedispersion_amd import check, load
+
+    n = args.read_calibration // 16 * 16
+    x = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    sink = torch.empty(2097152, dtype=torch.float32, device="cuda")
+    s = torch.cuda.current_stream().cuda_stream
+
+    def read():
+        check(load().dd_read_probe(x.data_ptr(), n, sink.data_ptr(), s))
+
+    for _ in range(2):
+        read()
+    torch.cuda.synchronize()
+    a, b = events(torch, 2)
+    a.record()
+    for _ in range(args.steps):
+        read()
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / args.steps
+    print(json.dumps({"read_bytes": n, "steps": args.steps, "ms_per_read": ms,
+                      "GB_per_s_read": n / (ms / 1e3) / 1e9}), flush=True)
 
 
 def copy_calibration(args, torch):
@@ -542,6 +573,9 @@ def main():
 
     if args.copy_calibration:
         copy_calibration(args, torch)
+        return
+    if args.read_calibration:
+        read_calibration(args, torch)
         return
     if args.workload == "batched-beams":
         line, w, mode = run_batched(args, torch)

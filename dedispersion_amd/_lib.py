@@ -177,6 +177,7 @@ PROTOTYPES = {
     "dd_subband_step_two": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_void_p]),
     "dd_detect_peaks": (c_int, [c_void_p, c_uint64, c_uint32, c_uint64, c_uint32, c_void_p, c_void_p]),
     "dd_detect_peaks_batch": (c_int, [POINTER(DDDetectJob), c_uint32, c_uint32, c_void_p]),
+    "dd_read_probe": (c_int, [c_void_p, c_uint64, c_void_p, c_void_p]),
     "dd_tune_time": (
         c_int,
         [c_void_p, c_int, POINTER(DDConf), c_void_p, c_void_p, c_uint32, c_uint32, c_void_p, POINTER(c_float),

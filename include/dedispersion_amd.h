@@ -540,6 +540,16 @@ dd_status dd_synth_inject_pulse(dd_plan *plan, uint8_t *d_beam_input, uint64_t p
                                 uint32_t row_len, float dm_value, uint32_t t0, uint32_t width,
                                 uint32_t amplitude, void *stream);
 
+/* ---- measurement ----
+ *
+ * No counterpart in the reference.  Reads `bytes` (a multiple of 16) of a
+ * 16-byte aligned device buffer with coalesced 16-byte loads and writes one
+ * partial sum per thread to d_sink (DD_READ_PROBE_SINK_FLOATS floats): the
+ * read bandwidth a streaming kernel reaches on the device, against which the
+ * read-bound kernels are measured beside the nominal HBM peak. */
+#define DD_READ_PROBE_SINK_FLOATS 2097152u
+dd_status dd_read_probe(const void *d_data, uint64_t bytes, float *d_sink, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

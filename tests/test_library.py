@@ -161,3 +161,30 @@ def test_cpp_cpu_functions_match_oracle(host_check, tmp_path, case):
         np.testing.assert_array_equal(raw(name + "_out", np.float32).reshape(want.shape), want, err_msg=name)
     want = orc[8].step_two(raw("two_in", np.float32))
     np.testing.assert_array_equal(raw("two_out", np.float32).reshape(want.shape), want, err_msg="step two")
+
+
+def test_peak_search_arguments_are_checked_on_the_host():
+    """dd_detect_peaks, dd_detect_peaks_batch and dd_read_probe refuse bad
+    arguments before any device call (so these run without a GPU)."""
+    from dedispersion_amd import DDError, detect_peaks, detect_peaks_batch
+
+    fake = 1 << 20  # never dereferenced: every call below fails its checks first
+    with pytest.raises(DDError, match="nr_widths"):
+        detect_peaks(fake, 4, 64, 64, 8, fake)
+    with pytest.raises(DDError, match="nr_samples"):
+        detect_peaks(fake, 4, 65, 64, 3, fake)
+    with pytest.raises(DDError, match="null argument"):
+        detect_peaks(0, 4, 64, 64, 3, fake)
+    with pytest.raises(DDError, match="too large"):
+        detect_peaks(fake, 1, 0xFFFFFFF0, 0xFFFFFFF0, 3, fake)
+    detect_peaks(fake, 0, 64, 64, 3, fake)  # no rows: nothing to do
+    detect_peaks_batch([], 7)  # no sets: nothing to do
+    with pytest.raises(DDError, match="nr_widths"):
+        detect_peaks_batch([(fake, 1, 64, 64, fake)], 0)
+    with pytest.raises(DDError, match="job 2: need 0 < nr_samples <= pitch"):
+        detect_peaks_batch([(fake, 1, 64, 64, fake), (0, 0, 0, 0, 0), (fake, 1, 64, 63, fake)], 7)
+    lib = _lib.load()
+    sink = fake
+    assert lib.dd_read_probe(fake + 4, 1024, sink, None) == 1   # misaligned buffer
+    assert lib.dd_read_probe(fake, 1000, sink, None) == 1       # not whole 16-byte pieces
+    assert lib.dd_read_probe(fake, 1024, 0, None) == 1          # no sink

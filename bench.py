@@ -376,6 +376,22 @@ def run_batched(args, torch):
     dominant = max(avg, key=avg.get)
     achieved = step_bytes[dominant] / (avg[dominant] / 1e3)
     total_bytes = bs.bytes_moved()
+
+    # a plain device copy (torch copy_) moving as many bytes as the dominant
+    # kernel, half read and half written: the practical ceiling of a kernel
+    # that reads and writes HBM, beside the 8 TB/s peak
+    src = torch.empty(step_bytes[dominant] // 2, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    a, b = events(torch, 2)
+    a.record()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize()
+    copy_rate = 2 * src.numel() / (a.elapsed_time(b) / args.steps / 1e3)
+    del src, dst
     reference_bytes = bs.bytes_moved(intermediate_bytes=4)  # float intermediate, the reference data flow
     value = units / (ms / 1e3)
     dms = sorted({w.obs.nr_dms_subbanding * w.obs.nr_dms for w in ws})
@@ -425,7 +441,10 @@ def run_batched(args, torch):
         "roofline": {"bound": "hbm", "kernel": names[dominant], "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": measured_traffic(workload, names[dominant], confs[names[dominant]]),
-                     "algorithmic_bytes": step_bytes[dominant]},
+                     "algorithmic_bytes": step_bytes[dominant],
+                     "copy_ceiling": {"GB_per_s": copy_rate / 1e9, "frac": achieved / copy_rate,
+                                      "path_frac": total_bytes / (ms / 1e3) / copy_rate,
+                                      "what": "a plain device copy (torch copy_) moving the dominant kernel's bytes"}},
         "wall_s": wall,
     }
     return line, ws, "two-step"

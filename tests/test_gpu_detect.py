@@ -222,6 +222,24 @@ def test_batch_rejects_bad_sets(torch_cuda):
         detect_peaks_batch([(0, 1, 64, 64, x.data_ptr())], 3)
 
 
+@pytest.mark.parametrize("nbytes", [16, 1 << 20, (1 << 26) + 48])
+def test_read_probe_reads_every_byte_once(torch_cuda, nbytes):
+    """The read probe behind the bench's read ceiling: its per-thread partial
+    sums add up to the buffer's sum (small integers: every sum exact), so it
+    reads every 16-byte piece exactly once."""
+    from dedispersion_amd import check, load
+
+    torch = torch_cuda
+    n = nbytes // 4
+    x = (torch.arange(n, device="cuda", dtype=torch.int64) % 7).to(torch.float32)
+    sink = torch.full((2097152,), -1.0, dtype=torch.float32, device="cuda")
+    check(load().dd_read_probe(x.data_ptr(), nbytes, sink.data_ptr(), stream_of(torch)))
+    torch.cuda.synchronize()
+    got = sink.double().sum().item()
+    want = float((torch.arange(n, dtype=torch.int64) % 7).sum().item())
+    assert got == want, (got, want)
+
+
 def test_rejects_bad_arguments(torch_cuda):
     from dedispersion_amd import DDError
 

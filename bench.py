@@ -496,6 +496,7 @@ def run_detect(args, torch):
 
     ms, wall = timed(step)
     per_beam_ms, _ = timed(per_beam)
+    cpu = None if args.no_cpu_baseline else detect_cpu_baseline(torch, bs, jobs, args.cpu_seconds / 2)
     # the same bytes read by a plain streaming kernel: the practical ceiling
     from dedispersion_amd import check, load
 
@@ -530,7 +531,45 @@ def run_detect(args, torch):
                                       "what": "dd_read_probe streaming the same number of bytes"}},
         "wall_s": wall,
     }
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
     return line, None, "detect"
+
+
+def detect_cpu_baseline(torch, bs, jobs, seconds: float):
+    """The oracle's straight-loop peak search (one thread) over a bounded
+    sample of the same rows: the first rows of every beam, the same number
+    from each, sized to about `seconds` of CPU work.  The sampled rows' peaks
+    are also compared with the device's, record for record."""
+    import numpy as np
+
+    import oracle
+    from dedispersion_amd import PEAK_DTYPE
+
+    def rows_of(b, k):
+        ptr, r, n, pitch, peaks = jobs[b]
+        host = bs.outs[b][: k * pitch].view(k, pitch).cpu().numpy()
+        dev = peaks[: k * PEAK_DTYPE.itemsize].cpu().numpy().view(PEAK_DTYPE)
+        return host, n, dev
+
+    host, n, _ = rows_of(0, 2)
+    t = time.perf_counter()
+    oracle.detect_peaks(host, n, 7)
+    rate = 2 * n / max(time.perf_counter() - t, 1e-4)  # samples per second
+    mean_samples = sum(n for _, _, n, _, _ in jobs) / len(jobs)
+    k = int(max(1, min(512, seconds * rate / (mean_samples * len(jobs)))))
+    spent, done, mismatched = 0.0, 0, 0
+    for b in range(len(jobs)):
+        k_b = min(k, jobs[b][1])
+        host, n, dev = rows_of(b, k_b)
+        t = time.perf_counter()
+        want = oracle.detect_peaks(host, n, 7)
+        spent += time.perf_counter() - t
+        done += k_b * n
+        mismatched += int(np.count_nonzero((want["sample"] != dev["sample"]) | (want["width"] != dev["width"])))
+    return {"value": done / spent, "unit": "DM-trial*samples/s", "cores": 1, "kind": "port",
+            "sample": f"the first {k} rows of each of the {len(jobs)} beams' outputs (the oracle's straight loops)",
+            "seconds": round(spent, 3), "sampled_rows_differing_from_device": mismatched}
 
 
 def run_single_plan(args, torch):

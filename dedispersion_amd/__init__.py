@@ -361,8 +361,8 @@ PEAK_DTYPE = np.dtype([("snr", "<f4"), ("mean", "<f4"), ("sigma", "<f4"), ("samp
 def detect_peaks(d_rows: int, nr_rows: int, nr_samples: int, pitch: int, nr_widths: int, d_peaks: int,
                  stream: int = 0) -> None:
     """Boxcar peak search (dd_detect_peaks) over nr_rows device rows of
-    nr_samples integer-valued floats, pitch floats apart; d_peaks receives
-    nr_rows dd_peak records (PEAK_DTYPE)."""
+    nr_samples non-negative integer-valued floats (below 2^24), pitch floats
+    apart; d_peaks receives nr_rows dd_peak records (PEAK_DTYPE)."""
     check(load().dd_detect_peaks(d_rows, nr_rows, nr_samples, pitch, nr_widths, d_peaks, stream or None))
 
 

@@ -209,6 +209,55 @@ def test_batch_of_more_sets_than_one_launch_holds(torch_cuda):
         compare(d.cpu().numpy().view(PEAK_DTYPE), oracle.detect_peaks(rows[:n], S, 5), f"S={S} pitch={pitch}")
 
 
+def test_random_sets_against_oracle(torch_cuda):
+    """25 seeded random batches of one to four sets: lengths 1 to 40000,
+    pitches up to four floats past the row, 0-24 rows, value ranges from
+    0-1 to 0-400000, and rows with a boxcar pulse, a constant value, two
+    equal impulses or a spike in the last sample; random width counts.
+    Batch, single calls and the oracle agree."""
+    torch = torch_cuda
+    rng = np.random.default_rng(31)
+    for case in range(25):
+        shapes, hosts = [], []
+        for _ in range(int(rng.integers(1, 5))):
+            S = int(rng.choice([rng.integers(1, 200), rng.integers(200, 9000), rng.integers(9000, 40000)]))
+            pitch = S + int(rng.integers(0, 5))
+            n = int(rng.integers(0, 25))
+            hi = int(rng.choice([2, 50, 5000, 400000]))
+            rows = rng.integers(0, hi, size=(max(n, 1), pitch)).astype(np.float32)
+            for r in range(n):
+                kind = int(rng.integers(0, 4))
+                if kind == 0 and S > 1:
+                    w = 1 << int(rng.integers(0, 7))
+                    t0 = int(rng.integers(0, max(1, S - w)))
+                    rows[r, t0:t0 + w] += 3 * hi
+                elif kind == 1:
+                    rows[r, :] = float(rng.integers(0, hi))
+                elif kind == 2 and S > 2:
+                    a, b = rng.choice(S, 2, replace=False)
+                    rows[r, :] = 0
+                    rows[r, a] = rows[r, b] = hi
+                else:
+                    rows[r, S - 1] += 5 * hi
+            shapes.append((n, S, pitch))
+            hosts.append(rows)
+        nw = int(rng.integers(1, 8))
+        dev = [torch.from_numpy(h.ravel()).cuda() for h in hosts]
+        peaks = [torch.zeros(max(n, 1) * PEAK_DTYPE.itemsize, dtype=torch.uint8, device="cuda") for n, _, _ in shapes]
+        detect_peaks_batch([(d.data_ptr(), n, S, pitch, pk.data_ptr()) for d, pk, (n, S, pitch) in
+                            zip(dev, peaks, shapes)], nw, stream_of(torch))
+        torch.cuda.synchronize()
+        for d, pk, h, (n, S, pitch) in zip(dev, peaks, hosts, shapes):
+            if n == 0:
+                continue
+            single = torch.zeros_like(pk)
+            detect_peaks(d.data_ptr(), n, S, pitch, nw, single.data_ptr(), stream_of(torch))
+            torch.cuda.synchronize()
+            got = pk.cpu().numpy().view(PEAK_DTYPE)
+            assert np.array_equal(got.view(np.uint8), single.cpu().numpy()), f"case {case} S={S}: batch vs single"
+            compare(got, oracle.detect_peaks(h[:n], S, nw), f"case {case} S={S} pitch={pitch} widths={nw}")
+
+
 def test_batch_rejects_bad_sets(torch_cuda):
     from dedispersion_amd import DDError
 

@@ -36,6 +36,11 @@
 // first run (68 starts) where it occurs; at the row's end the wavefront
 // reduces them, picks the width of the highest ratio, and finds the first
 // start inside that width's first run by rescanning those 131 samples.
+//
+// One launch takes up to 64 sets of rows (the outputs of as many beams,
+// lengths and pitches free to differ) from its kernel arguments and deals
+// all their rows to 2048 wavefronts round robin, so a batch of beams pays
+// one launch's ramp and drain instead of one per beam.
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -52,7 +57,6 @@ using ddamd::set_error;
 using ddamd::staged::make_rsrc;
 using ddamd::staged::records_from_floats;
 using ddamd::staged::u32x4;
-using ddamd::staged::uniform;
 
 constexpr int kWidths = 7;                    // widths 1 .. 64
 constexpr int kMaxWidth = 1 << (kWidths - 1);
